@@ -1,0 +1,132 @@
+/*
+ * rv_sparse_hip.h — MI355X (gfx950) extensions to the rv-sparse C API.
+ *
+ * None of these has a counterpart in the reference, whose operands are always
+ * host arrays and whose work always runs on the calling thread. They exist
+ * because a GPU caller needs three things the reference API cannot express:
+ * operands already resident in device memory, a stream to order the work on,
+ * and a way to see which accumulator kernels the analysis chose. hip types
+ * are passed as void * so this header needs no HIP include.
+ */
+#ifndef RV_SPARSE_HIP_H
+#define RV_SPARSE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "rv_sparse.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Where the descriptor's operand arrays live. */
+typedef enum
+{
+    /* A, B and C arrays are host memory (reference semantics). Inputs are
+     * copied to the device by work_estimation/compute; C is copied back. */
+    RVSP_MEMORY_HOST = 0,
+    /* A, B and C arrays are device pointers of the current HIP device. No
+     * host copies are made; compute writes C in place. */
+    RVSP_MEMORY_DEVICE = 1
+} rvsp_memory_t;
+
+/* Selects operand memory for this descriptor. Must precede work_estimation
+ * (RVSP_ERROR_INVALID_ARGUMENT afterwards, as for set_algo). */
+rvsp_status_t rvsp_spgemm_set_memory(rvsp_spgemm_descr_t descr,
+                                     rvsp_memory_t memory);
+
+/* Orders all of the descriptor's device work on `hip_stream` (a
+ * hipStream_t); NULL returns to the descriptor's own stream. */
+rvsp_status_t rvsp_spgemm_set_stream(rvsp_spgemm_descr_t descr,
+                                     void *hip_stream);
+
+/* The hipStream_t the descriptor enqueues on (creating its own stream on
+ * first use). */
+rvsp_status_t rvsp_spgemm_get_stream(rvsp_spgemm_descr_t descr,
+                                     void **hip_stream_out);
+
+/* async != 0: in RVSP_MEMORY_DEVICE mode, compute returns once its kernels
+ * are enqueued instead of waiting for them (the caller synchronises the
+ * stream). work_estimation always waits, because it reports c_nnz. Host
+ * mode always waits. */
+rvsp_status_t rvsp_spgemm_set_async(rvsp_spgemm_descr_t descr, int async);
+
+/* Phase ids used by the plan queries below. */
+typedef enum
+{
+    RVSP_PHASE_COUNT = 0,   /* symbolic nnz count */
+    RVSP_PHASE_FILL = 1,    /* sorted column indices */
+    RVSP_PHASE_NUMERIC = 2  /* values */
+} rvsp_phase_t;
+
+/* Accumulator kinds a bin can be served by. */
+typedef enum
+{
+    RVSP_KIND_NONE = 0,    /* empty rows, nothing to do */
+    RVSP_KIND_DIRECT = 1,  /* one contributing B row: scaled copy */
+    RVSP_KIND_HASH = 2,    /* LDS hash table, sub-wave group per row */
+    RVSP_KIND_SEARCH = 3,  /* LDS sorted columns + binary search */
+    RVSP_KIND_BITMAP = 4,  /* LDS bitmap over the row's column span */
+    RVSP_KIND_BLOCK = 5,   /* workgroup per row, LDS, waves own column ranges */
+    RVSP_KIND_GLOBAL = 6   /* workgroup per row, dense global scratch */
+} rvsp_kind_t;
+
+/* One bin of the analysed plan. */
+typedef struct
+{
+    int32_t phase;     /* rvsp_phase_t */
+    int32_t bin;
+    int32_t kind;      /* rvsp_kind_t */
+    int32_t group;     /* lanes per row */
+    int32_t cap;       /* table slots / columns / bitmap bits */
+    int32_t rows;      /* rows assigned to the bin */
+    int32_t lds_bytes; /* dynamic LDS per workgroup */
+    int32_t bound;     /* largest size metric the bin admits */
+} rvsp_bin_info_t;
+
+/* Copies up to `max` bins of the plan built by the last work_estimation
+ * (count, fill and numeric phases, empty bins included) and the number
+ * available in *n_out. */
+rvsp_status_t rvsp_spgemm_get_plan(rvsp_spgemm_descr_t descr,
+                                   rvsp_bin_info_t *out, int32_t max,
+                                   int32_t *n_out);
+
+/* A per-bin override applied by the next work_estimation. kind/group/cap of
+ * 0 keep the table's value. Whatever is asked, the exact symbolic count has
+ * the last word: a choice whose table cannot hold the bin's largest row, or
+ * whose shape cannot launch, turns the bin into RVSP_KIND_GLOBAL. Output bits
+ * never depend on these choices. */
+typedef struct
+{
+    int32_t phase;
+    int32_t bin;
+    int32_t kind;
+    int32_t group;
+    int32_t cap;
+} rvsp_bin_choice_t;
+
+/* Replaces the descriptor's override list (n == 0 clears it). Must precede
+ * work_estimation. */
+rvsp_status_t rvsp_spgemm_set_bin_choices(rvsp_spgemm_descr_t descr,
+                                          const rvsp_bin_choice_t *choices,
+                                          int32_t n);
+
+/* Device memory helpers for callers without a HIP binding of their own
+ * (for example ctypes). kind: 1 host->device, 2 device->host,
+ * 3 device->device. */
+rvsp_status_t rvsp_hip_malloc(void **ptr, size_t bytes);
+rvsp_status_t rvsp_hip_free(void *ptr);
+rvsp_status_t rvsp_hip_memcpy(void *dst, const void *src, size_t bytes,
+                              int kind);
+rvsp_status_t rvsp_hip_synchronize(void);
+
+/* Number of usable gfx950 devices (0 when none; never an error for "no
+ * GPU"). */
+int32_t rvsp_hip_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RV_SPARSE_HIP_H */

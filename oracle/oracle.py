@@ -1,0 +1,257 @@
+"""CPU oracle for the SpGEMM hot path — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
+this module, and only as the checker or the CPU baseline. The product
+library never loads it.
+
+Two CPU implementations sit behind it:
+  liboracle.so          this project's restatement of the reference scalar
+                        algorithm (spgemm_oracle.c; cites the reference lines
+                        it follows)
+  _ref/librvsp_ref.so   the reference's own C sources compiled as they lie
+                        (oracle/Makefile `ref`); present only where the
+                        reference tree was available at build time
+
+Pinning (see tests/test_oracle.py): both libraries must agree bit for bit
+on seeded inputs and with the committed golden digests; the six hand fixtures
+of the reference's tests/csr_fixtures.h:42-154; brute-force dense products on
+tiny inputs; the closed-form nnz of the 2-D Laplacian squared; and planted
+mistakes (FMA, reversed order) must be caught.
+"""
+import ctypes as C
+import hashlib
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "librvsp_ref.so")
+
+_P = C.c_void_p
+_I32 = C.c_int32
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else C.c_void_p(0)
+
+
+_olib = None
+
+
+def olib():
+    global _olib
+    if _olib is None:
+        if not os.path.exists(ORACLE_LIB):
+            raise RuntimeError(f"{ORACLE_LIB} missing: run `make -C oracle`")
+        h = C.CDLL(ORACLE_LIB)
+        h.oracle_symbolic_count.argtypes = [_I32, _I32, _I32] + [_P] * 7
+        h.oracle_symbolic_fill.argtypes = [_I32, _I32, _I32] + [_P] * 6
+        h.oracle_numeric_f32.argtypes = [_I32, _I32, _I32] + [_P] * 9
+        h.oracle_numeric_f32_variant.argtypes = [_I32, _I32, _I32] + [_P] * 9 + [C.c_int]
+        h.oracle_numeric_f64.argtypes = [_I32, _I32, _I32] + [_P] * 10
+        for f in ("oracle_symbolic_count", "oracle_symbolic_fill", "oracle_numeric_f32",
+                  "oracle_numeric_f32_variant", "oracle_numeric_f64"):
+            getattr(h, f).restype = C.c_int
+        _olib = h
+    return _olib
+
+
+class OracleOverflow(RuntimeError):
+    pass
+
+
+def symbolic(A, B, r0=0, r1=None):
+    """Block-local (row_ptr, op_counts) of C's rows [r0, r1)."""
+    r1 = A.rows if r1 is None else r1
+    rp = np.zeros(r1 - r0 + 1, dtype=np.int32)
+    ops = np.zeros(r1 - r0, dtype=np.int64)
+    tot = C.c_int64()
+    st = olib().oracle_symbolic_count(r0, r1, B.cols, _ptr(A.row_ptr), _ptr(A.col_idx),
+                                      _ptr(B.row_ptr), _ptr(B.col_idx), _ptr(rp),
+                                      _ptr(ops), C.byref(tot))
+    if st == -6:
+        raise OracleOverflow("nnz(C) exceeds INT32_MAX")
+    if st:
+        raise MemoryError("oracle allocation failed")
+    return rp, ops
+
+
+def spgemm(A, B, r0=0, r1=None, variant=0):
+    """(row_ptr, col_idx, values) of C's rows [r0, r1), reference semantics."""
+    r1 = A.rows if r1 is None else r1
+    rp, _ = symbolic(A, B, r0, r1)
+    nnz = int(rp[-1])
+    ci = np.zeros(max(nnz, 1), dtype=np.int32)
+    val = np.zeros(max(nnz, 1), dtype=np.float32)
+    h = olib()
+    args = (r0, r1, B.cols, _ptr(A.row_ptr), _ptr(A.col_idx))
+    if h.oracle_symbolic_fill(*args, _ptr(B.row_ptr), _ptr(B.col_idx), _ptr(rp), _ptr(ci)):
+        raise MemoryError("oracle allocation failed")
+    if h.oracle_numeric_f32_variant(*args, _ptr(A.values), _ptr(B.row_ptr), _ptr(B.col_idx),
+                                    _ptr(B.values), _ptr(rp), _ptr(ci), _ptr(val), variant):
+        raise MemoryError("oracle allocation failed")
+    return rp, ci[:nnz], val[:nnz]
+
+
+def spgemm_f64(A, B, rp, ci, r0=0):
+    """FP64 values and sum_k |a_ik b_kj| for the structure (rp, ci) of rows
+    [r0, r0 + len(rp) - 1)."""
+    r1 = r0 + rp.size - 1
+    n = max(int(rp[-1]), 1)
+    v = np.zeros(n, dtype=np.float64)
+    mag = np.zeros(n, dtype=np.float64)
+    ci = np.ascontiguousarray(ci, dtype=np.int32)
+    if olib().oracle_numeric_f64(r0, r1, B.cols, _ptr(A.row_ptr), _ptr(A.col_idx),
+                                 _ptr(A.values), _ptr(B.row_ptr), _ptr(B.col_idx),
+                                 _ptr(B.values), _ptr(rp), _ptr(ci), _ptr(v), _ptr(mag)):
+        raise MemoryError("oracle allocation failed")
+    return v[:int(rp[-1])], mag[:int(rp[-1])]
+
+
+# ---------------------------------------------------------------------------
+# The reference library built from its own sources.
+
+_rlib = None
+
+
+def ref_available():
+    return os.path.exists(REF_LIB)
+
+
+def rlib():
+    global _rlib
+    if _rlib is None:
+        h = C.CDLL(REF_LIB)
+        h.rvsp_spgemm_scalar_f32.argtypes = [_I32, _I32, _I32] + [_P] * 7 + [C.c_size_t] + \
+            [C.POINTER(_P)] * 3 + [C.POINTER(_I32), _P]
+        h.rvsp_spgemm_scalar_f32.restype = C.c_int
+        h.rvsp_spgemm_buffer_size.argtypes = [_I32, C.POINTER(C.c_size_t)]
+        h.rvsp_spgemm_buffer_size.restype = C.c_int
+        _rlib = h
+    return _rlib
+
+
+_libc = C.CDLL(None)
+_libc.free.argtypes = [_P]
+
+
+def ref_spgemm(A, B, want_ops=False):
+    """The reference's raw scalar kernel (gustavson_core_f32.inc:66-194)."""
+    h = rlib()
+    ws_bytes = C.c_size_t()
+    h.rvsp_spgemm_buffer_size(B.cols, C.byref(ws_bytes))
+    ws = np.zeros(ws_bytes.value, dtype=np.uint8)
+    rp, ci, val = _P(), _P(), _P()
+    nnz = _I32()
+    ops = np.zeros(A.rows, dtype=np.int64) if want_ops else None
+    st = h.rvsp_spgemm_scalar_f32(A.rows, A.cols, B.cols, _ptr(A.row_ptr), _ptr(A.col_idx),
+                                  _ptr(A.values), _ptr(B.row_ptr), _ptr(B.col_idx),
+                                  _ptr(B.values), _ptr(ws), ws_bytes.value, C.byref(rp),
+                                  C.byref(ci), C.byref(val), C.byref(nnz), _ptr(ops))
+    if st != 0:
+        raise RuntimeError(f"reference kernel returned {st}")
+    n = nnz.value
+    out_rp = np.ctypeslib.as_array((C.c_int32 * (A.rows + 1)).from_address(rp.value)).copy()
+    out_ci = np.ctypeslib.as_array((C.c_int32 * max(n, 1)).from_address(ci.value))[:n].copy()
+    out_v = np.ctypeslib.as_array((C.c_float * max(n, 1)).from_address(val.value))[:n].copy()
+    for p in (rp, ci, val):
+        _libc.free(p)
+    return (out_rp, out_ci, out_v, ops) if want_ops else (out_rp, out_ci, out_v)
+
+
+# ---------------------------------------------------------------------------
+
+def digest(rp, ci, val):
+    """SHA-256 of the little-endian bytes of row_ptr, col_idx and values."""
+    h = hashlib.sha256()
+    for a, dt in ((rp, "<i4"), (ci, "<i4"), (val, "<f4")):
+        h.update(np.ascontiguousarray(a, dtype=dt).tobytes())
+    return h.hexdigest()
+
+
+def dense_product_structure(A, B):
+    """Brute force for tiny inputs: pattern of |A|*|B| with ones as values
+    (no cancellation possible), as (row_ptr, col_idx)."""
+    da = np.zeros((A.rows, A.cols))
+    db = np.zeros((B.rows, B.cols))
+    ra = np.repeat(np.arange(A.rows), np.diff(A.row_ptr))
+    rb = np.repeat(np.arange(B.rows), np.diff(B.row_ptr))
+    da[ra, A.col_idx] = 1.0
+    db[rb, B.col_idx] = 1.0
+    mask = (da @ db) > 0
+    rp = np.zeros(A.rows + 1, dtype=np.int32)
+    rp[1:] = np.cumsum(mask.sum(axis=1))
+    return rp, np.nonzero(mask)[1].astype(np.int32)
+
+
+# ---------------------------------------------------------------------------
+# CPU baselines for bench.py: numeric-only multiplies through a descriptor.
+
+def _ref_descr_api():
+    h = rlib()
+    if getattr(h, "_descr_ready", False):
+        return h
+    P = C.c_void_p
+    h.rvsp_spgemm_descr_create.argtypes = [C.POINTER(P)]
+    h.rvsp_spgemm_descr_destroy.argtypes = [P]
+    h.rvsp_spgemm_work_estimation.argtypes = [P, P, P, C.POINTER(C.c_size_t), C.POINTER(_I32)]
+    h.rvsp_spgemm_compute.argtypes = [P, P, P, P, P]
+    for f in ("rvsp_spgemm_descr_create", "rvsp_spgemm_work_estimation", "rvsp_spgemm_compute"):
+        getattr(h, f).restype = C.c_int
+    h._descr_ready = True
+    return h
+
+
+def time_reference_numeric(A, B, reps, struct_cls, values_list=None):
+    """Seconds per numeric-only compute of the reference library's
+    descriptor path (src/core/spgemm_descr.c compute with the fill skipped),
+    one thread. Returns (times, nnz_c)."""
+    import time
+    h = _ref_descr_api()
+    d = C.c_void_p()
+    h.rvsp_spgemm_descr_create(C.byref(d))
+    a = struct_cls(A.rows, A.cols, A.nnz, A.row_ptr.ctypes.data, A.col_idx.ctypes.data,
+                   A.values.ctypes.data, 3, 0, 0)
+    b = struct_cls(B.rows, B.cols, B.nnz, B.row_ptr.ctypes.data, B.col_idx.ctypes.data,
+                   B.values.ctypes.data, 3, 0, 0)
+    ws_bytes, nnz = C.c_size_t(), _I32()
+    if h.rvsp_spgemm_work_estimation(d, C.byref(a), C.byref(b), C.byref(ws_bytes), C.byref(nnz)):
+        raise RuntimeError("reference work_estimation failed")
+    n = max(nnz.value, 1)
+    rp = np.zeros(A.rows + 1, np.int32)
+    ci = np.zeros(n, np.int32)
+    v = np.zeros(n, np.float32)
+    ws = np.zeros(ws_bytes.value, np.uint8)
+    cm = struct_cls(A.rows, B.cols, nnz.value, rp.ctypes.data, ci.ctypes.data, v.ctypes.data,
+                    3, 0, 0)
+    h.rvsp_spgemm_compute(d, C.byref(a), C.byref(b), C.byref(cm), _ptr(ws))  # fill
+    times = []
+    for r in range(reps):
+        if values_list is not None:
+            a.values = values_list[r % len(values_list)].ctypes.data
+            if A is B:
+                b.values = a.values
+        t0 = time.perf_counter()
+        h.rvsp_spgemm_compute(d, C.byref(a), C.byref(b), C.byref(cm), _ptr(ws))
+        times.append(time.perf_counter() - t0)
+    h.rvsp_spgemm_descr_destroy(d)
+    return times, nnz.value
+
+
+def time_port_numeric(A, B, reps):
+    """Seconds per numeric pass of this project's oracle restatement."""
+    import time
+    rp, _ = symbolic(A, B)
+    ci = np.zeros(max(int(rp[-1]), 1), np.int32)
+    olib().oracle_symbolic_fill(0, A.rows, B.cols, _ptr(A.row_ptr), _ptr(A.col_idx),
+                                _ptr(B.row_ptr), _ptr(B.col_idx), _ptr(rp), _ptr(ci))
+    v = np.zeros_like(ci, dtype=np.float32)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        olib().oracle_numeric_f32(0, A.rows, B.cols, _ptr(A.row_ptr), _ptr(A.col_idx),
+                                  _ptr(A.values), _ptr(B.row_ptr), _ptr(B.col_idx),
+                                  _ptr(B.values), _ptr(rp), _ptr(ci), _ptr(v))
+        times.append(time.perf_counter() - t0)
+    return times, int(rp[-1])

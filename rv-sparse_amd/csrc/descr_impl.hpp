@@ -1,0 +1,101 @@
+// descr_impl.hpp — layout of the SpGEMM descriptor and the internal calls
+// shared by descr.cpp, the tuner, the learned selector and the batch path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "kernels.hpp"
+#include "plan.hpp"
+#include "rv_sparse.h"
+#include "rv_sparse_hip.h"
+
+namespace rvsp {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+rvsp_status_t ensure(DevBuf& b, size_t bytes);
+void release(DevBuf& b);
+rvsp_status_t device_ready();
+
+// One phase's bin table plus the rows assigned to each bin.
+struct PhasePlan {
+  PhaseTable table;
+  std::vector<int> offsets;  // host copy, nbins + 1 entries
+  DevBuf rows;               // row ids grouped by bin, ascending within a bin
+};
+
+}  // namespace rvsp
+
+// The reference's descriptor (src/core/spgemm_descr.c:19-37: algo,
+// estimated flag, c_row_ptr, c_nnz, op counts, the filled_into reuse key,
+// captured dimensions, workspace size) plus the device state of the HIP
+// path. op counts live on the device and are copied out on request.
+struct rvsp_spgemm_descr {
+  rvsp_spgemm_algo_t algo = RVSP_SPGEMM_ALGO_DEFAULT;
+  int estimated = 0;
+  int32_t* c_row_ptr = nullptr;  // host copy (host memory mode only)
+  int32_t c_nnz = 0;
+  const int32_t* filled_into = nullptr;
+  const int32_t* rowptr_into = nullptr;  // device mode: C.row_ptr already written
+  bool dev_cols_valid = false;           // host mode: c_ci_dev holds the columns
+  int32_t a_rows = 0, a_cols = 0, a_nnz = 0;
+  int32_t b_rows = 0, b_cols = 0, b_nnz = 0;
+  size_t workspace_bytes = 0;
+
+  rvsp_memory_t memory = RVSP_MEMORY_HOST;
+  hipStream_t own_stream = nullptr;
+  hipStream_t user_stream = nullptr;
+  int async = 0;
+  std::vector<rvsp_bin_choice_t> choices;
+
+  // Host-mode staging of the operands.
+  rvsp::DevBuf a_rp, a_ci, a_val, b_rp, b_ci, b_val;
+  bool b_struct_alias = false, b_val_alias = false;
+
+  // Analysis and symbolic state.
+  rvsp::DevBuf ops, lo, hi, maxb, long_list, n_long;
+  rvsp::DevBuf cnt, c_rp, scan_tiles, total;
+  rvsp::DevBuf bin_of, tile_hist, tile_base, bin_offsets;
+  int tile_rows = 0, ntiles = 0;
+  rvsp::PhasePlan count, fill, numeric;
+
+  // Host-mode device copy of C.
+  rvsp::DevBuf c_ci_dev, c_val_dev;
+
+  // Global scratch of GLOBAL bins.
+  rvsp::DevBuf gbits, gdense;
+  rvsp::GlobalSlots gs_bits, gs_dense;
+
+  // Learned selector features (per numeric bin).
+  rvsp::DevBuf learn_feat;
+
+  // Captured numeric-only graph.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  const void* graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace rvsp {
+
+rvsp_status_t descr_stream(rvsp_spgemm_descr* d, hipStream_t* out);
+RowStats row_stats(rvsp_spgemm_descr* d);
+rvsp_status_t run_fill(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
+                       int* c_ci, hipStream_t s);
+rvsp_status_t run_numeric(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
+                          const int* c_ci, float* c_val, hipStream_t s);
+
+// Autotuned per-bin table (RVSP_SPGEMM_ALGO_HIP_TUNED).
+void apply_tuned_table(Strategy& st, rvsp_spgemm_descr* d);
+// Learned per-bin accumulator choice (RVSP_SPGEMM_ALGO_HIP_LEARNED), applied
+// to the fill and numeric plans after binning.
+rvsp_status_t apply_learned(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
+                            hipStream_t s);
+void destroy_graph(rvsp_spgemm_descr* d);
+
+}  // namespace rvsp

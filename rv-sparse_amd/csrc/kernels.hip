@@ -1,0 +1,920 @@
+// kernels.hip — gfx950 kernels of the CSR SpGEMM C = A * B (FP32).
+//
+// Row-wise Gustavson, as in the reference (src/kernels/spgemm/
+// gustavson_core_f32.inc:18-64): output row i is the union of the B rows
+// selected by A's row i. The GPU version splits the work into phases
+// (analysis, binning, count, scan, fill, numeric) and assigns every output
+// row to one accumulator kernel by its size.
+//
+// Bit-exactness. The reference accumulates C(i,j) as acc = acc + a*b with a
+// separately rounded product and sum, starting from +0.0f, taking the A
+// entries of row i in ascending position (accum_scalar_f32.c:67-70 under
+// -std=c11). Inside one B row every column appears once (canonical CSR), so
+// the products of one A entry never collide. Every numeric kernel here walks
+// A's row in order and lets lanes split only the B row of the current A
+// entry, or splits the output columns between waves; the additions into any
+// one C(i,j) therefore happen in the reference's order and the result is
+// bit-identical, whatever kernel a row is assigned to.
+#include <climits>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace rvsp {
+
+// ---------------------------------------------------------------------------
+// Small helpers.
+
+// Lane `l` of a G-lane group visits, in lockstep with the rest of the group,
+// every (A entry, B entry) product of A's row segment [a_s, a_e) and calls
+// f(col) for the B column of each product it owns. A entries are taken G at
+// a time; their B-row lengths are prefix-summed across the group and the
+// flattened products are dealt out round-robin (load-balanced search), so a
+// group stays busy whatever the mix of long and short B rows. Order of the
+// calls is unspecified: only for order-free work (count, fill).
+template <int G, class F>
+__device__ __forceinline__ void group_for_each_col(int a_s, int a_e,
+                                                   const int* __restrict__ a_ci,
+                                                   const int* __restrict__ b_rp,
+                                                   const int* __restrict__ b_ci,
+                                                   F&& f) {
+  const int l = threadIdx.x & (G - 1);
+  for (int base = a_s; base < a_e; base += G) {
+    const int ap = base + l;
+    int bs = 0, len = 0;
+    if (ap < a_e) {
+      const int k = a_ci[ap];
+      bs = b_rp[k];
+      len = b_rp[k + 1] - bs;
+    }
+    const int incl = group_incl_scan<G>(len);
+    const int total = __shfl(incl, G - 1, G);
+    const int excl = incl - len;
+    for (int p0 = 0; p0 < total; p0 += G) {
+      const int p = p0 + l;
+      // Owner: the last lane whose exclusive offset is <= p.
+      int j = 0;
+#pragma unroll
+      for (int step = G >> 1; step > 0; step >>= 1) {
+        const int e = __shfl(excl, j + step, G);
+        if (e <= p) j += step;
+      }
+      const int oe = __shfl(excl, j, G);
+      const int ob = __shfl(bs, j, G);
+      if (p < total) f(b_ci[ob + (p - oe)]);
+    }
+  }
+}
+
+// Whole-workgroup version of group_for_each_col (blockDim.x == kBlock).
+// `sh_excl` and `sh_bs` hold kBlock ints each, `tmp` kBlock / kWave ints.
+// Must be reached by every thread of the block.
+template <class F>
+__device__ __forceinline__ void block_for_each_col(int a_s, int a_e,
+                                                   const int* __restrict__ a_ci,
+                                                   const int* __restrict__ b_rp,
+                                                   const int* __restrict__ b_ci,
+                                                   int* sh_excl, int* sh_bs,
+                                                   int* tmp, F&& f) {
+  const int t = threadIdx.x;
+  for (int base = a_s; base < a_e; base += kBlock) {
+    const int ap = base + t;
+    int bs = 0, len = 0;
+    if (ap < a_e) {
+      const int k = a_ci[ap];
+      bs = b_rp[k];
+      len = b_rp[k + 1] - bs;
+    }
+    int total;
+    const int excl = block_excl_scan(len, tmp, &total);
+    sh_excl[t] = excl;
+    sh_bs[t] = bs;
+    __syncthreads();
+    for (int p = t; p < total; p += kBlock) {
+      int j = 0;
+#pragma unroll
+      for (int step = kBlock >> 1; step > 0; step >>= 1)
+        if (sh_excl[j + step] <= p) j += step;
+      f(b_ci[sh_bs[j] + (p - sh_excl[j])]);
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ int pick_bin(const BinRule& r, int metric, int span) {
+  for (int b = 2; b < r.nbins; ++b)
+    if (metric <= r.bound[b] && span <= r.span_max[b]) return b;
+  return r.nbins - 1;
+}
+
+// ---------------------------------------------------------------------------
+// Analysis: Op_i, span and longest B row per output row.
+
+constexpr int kLongA = 64;  // A rows longer than this are analysed by a wave
+
+__global__ __launch_bounds__(kBlock) void k_analyze(
+    int M, const int* __restrict__ a_rp, const int* __restrict__ a_ci,
+    const int* __restrict__ b_rp, const int* __restrict__ b_ci, RowStats st,
+    int* __restrict__ long_list, int* __restrict__ n_long) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  const int s = a_rp[i], e = a_rp[i + 1];
+  if (e - s > kLongA) {
+    long_list[atomicAdd(n_long, 1)] = i;
+    return;
+  }
+  long long o = 0;
+  int lo = INT_MAX, hi = -1, mb = 0;
+  for (int ap = s; ap < e; ++ap) {
+    const int k = a_ci[ap];
+    const int bs = b_rp[k], be = b_rp[k + 1];
+    const int len = be - bs;
+    if (len > 0) {
+      o += len;
+      lo = min(lo, b_ci[bs]);
+      hi = max(hi, b_ci[be - 1]);
+      mb = max(mb, len);
+    }
+  }
+  st.ops[i] = o;
+  st.lo[i] = lo;
+  st.hi[i] = hi;
+  st.maxb[i] = mb;
+}
+
+__global__ __launch_bounds__(kBlock) void k_analyze_long(
+    const int* __restrict__ a_rp, const int* __restrict__ a_ci,
+    const int* __restrict__ b_rp, const int* __restrict__ b_ci, RowStats st,
+    const int* __restrict__ long_list, const int* __restrict__ n_long) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int nwaves = (gridDim.x * blockDim.x) / kWave;
+  const int n = *n_long;
+  for (int t = wave; t < n; t += nwaves) {
+    const int i = long_list[t];
+    const int s = a_rp[i], e = a_rp[i + 1];
+    long long o = 0;
+    int lo = INT_MAX, hi = -1, mb = 0;
+    for (int ap = s + lane; ap < e; ap += kWave) {
+      const int k = a_ci[ap];
+      const int bs = b_rp[k], be = b_rp[k + 1];
+      const int len = be - bs;
+      if (len > 0) {
+        o += len;
+        lo = min(lo, b_ci[bs]);
+        hi = max(hi, b_ci[be - 1]);
+        mb = max(mb, len);
+      }
+    }
+    o = group_sum<kWave>(o);
+    lo = group_min<kWave>(lo);
+    hi = group_max<kWave>(hi);
+    mb = group_max<kWave>(mb);
+    if (lane == 0) {
+      st.ops[i] = o;
+      st.lo[i] = lo;
+      st.hi[i] = hi;
+      st.maxb[i] = mb;
+    }
+  }
+}
+
+hipError_t launch_analyze(const DevCsr& A, const DevCsr& B, RowStats st,
+                          int* long_list, int* n_long, hipStream_t s) {
+  if (A.rows <= 0) return hipSuccess;
+  hipError_t e = hipMemsetAsync(n_long, 0, sizeof(int), s);
+  if (e != hipSuccess) return e;
+  const int grid = (A.rows + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_analyze, dim3(grid), dim3(kBlock), 0, s, A.rows, A.rp,
+                     A.ci, B.rp, B.ci, st, long_list, n_long);
+  hipLaunchKernelGGL(k_analyze_long, dim3(1024), dim3(kBlock), 0, s, A.rp,
+                     A.ci, B.rp, B.ci, st, long_list, n_long);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Classification.
+
+__global__ __launch_bounds__(kBlock) void k_classify_count(
+    int M, const int* __restrict__ a_rp, RowStats st, BinRule r,
+    unsigned char* __restrict__ bin_of, int* __restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  const long long o = st.ops[i];
+  int b = 0;
+  if (o == 0) {
+    cnt[i] = 0;
+  } else if ((long long)st.maxb[i] == o) {
+    // Every product comes from one B row, whose columns are distinct.
+    cnt[i] = (int)o;
+  } else {
+    const int span = st.hi[i] - st.lo[i] + 1;
+    const int ub = (int)min(o, (long long)span);
+    b = pick_bin(r, ub, span);
+  }
+  bin_of[i] = (unsigned char)b;
+}
+
+__global__ __launch_bounds__(kBlock) void k_classify_nnz(
+    int M, RowStats st, const int* __restrict__ c_rp, BinRule r,
+    unsigned char* __restrict__ bin_of) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  const int n = c_rp[i + 1] - c_rp[i];
+  int b;
+  if (n == 0)
+    b = 0;
+  else if ((long long)st.maxb[i] == st.ops[i])
+    b = 1;
+  else
+    b = pick_bin(r, n, st.hi[i] - st.lo[i] + 1);
+  bin_of[i] = (unsigned char)b;
+}
+
+hipError_t launch_classify_count(const DevCsr& A, RowStats st, const BinRule& r,
+                                 unsigned char* bin_of, int* cnt, hipStream_t s) {
+  const int grid = (A.rows + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_classify_count, dim3(grid), dim3(kBlock), 0, s, A.rows,
+                     A.rp, st, r, bin_of, cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_classify_nnz(const DevCsr& A, RowStats st, const int* c_rp,
+                               const BinRule& r, unsigned char* bin_of,
+                               hipStream_t s) {
+  const int grid = (A.rows + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_classify_nnz, dim3(grid), dim3(kBlock), 0, s, A.rows, st,
+                     c_rp, r, bin_of);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Stable partition of rows by bin (three passes: per-tile histogram, scan of
+// the histograms, ordered scatter). Rows keep ascending order inside a bin,
+// so neighbouring groups work on neighbouring rows.
+
+__global__ __launch_bounds__(kBlock) void k_part_hist(
+    int M, int nb, int tile_rows, const unsigned char* __restrict__ bin_of,
+    int* __restrict__ tile_hist) {
+  __shared__ int h[kMaxBins];
+  const int lane = threadIdx.x & (kWave - 1);
+  if (threadIdx.x < kMaxBins) h[threadIdx.x] = 0;
+  __syncthreads();
+  const int t0 = blockIdx.x * tile_rows;
+  const int t1 = min(M, t0 + tile_rows);
+  for (int c = t0; c < t1; c += kBlock) {
+    const int i = c + threadIdx.x;
+    const int b = (i < t1) ? (int)bin_of[i] : -1;
+    for (int bb = 0; bb < nb; ++bb) {
+      const unsigned long long m = __ballot(b == bb);
+      if (lane == 0 && m) atomicAdd(&h[bb], __popcll(m));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kMaxBins)
+    tile_hist[blockIdx.x * kMaxBins + threadIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kBlock) void k_part_scan(
+    int ntiles, int nb, const int* __restrict__ tile_hist,
+    int* __restrict__ tile_base, int* __restrict__ bin_offsets) {
+  __shared__ int tmp[kBlock / kWave];
+  __shared__ int tot[kMaxBins];
+  for (int b = 0; b < nb; ++b) {
+    int running = 0;
+    for (int t0 = 0; t0 < ntiles; t0 += kBlock) {
+      const int t = t0 + threadIdx.x;
+      const int v = (t < ntiles) ? tile_hist[t * kMaxBins + b] : 0;
+      int total;
+      const int ex = block_excl_scan(v, tmp, &total);
+      if (t < ntiles) tile_base[t * kMaxBins + b] = running + ex;
+      running += total;
+    }
+    if (threadIdx.x == 0) tot[b] = running;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int b = 0; b < nb; ++b) {
+      bin_offsets[b] = acc;
+      acc += tot[b];
+    }
+    bin_offsets[nb] = acc;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_part_scatter(
+    int M, int nb, int tile_rows, const unsigned char* __restrict__ bin_of,
+    const int* __restrict__ tile_base, const int* __restrict__ bin_offsets,
+    int* __restrict__ rows_out) {
+  __shared__ int run[kMaxBins];
+  __shared__ int wcnt[kBlock / kWave][kMaxBins];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x >> 6;
+  if (threadIdx.x < nb)
+    run[threadIdx.x] =
+        bin_offsets[threadIdx.x] + tile_base[blockIdx.x * kMaxBins + threadIdx.x];
+  __syncthreads();
+  const int t0 = blockIdx.x * tile_rows;
+  const int t1 = min(M, t0 + tile_rows);
+  const unsigned long long below = __lanemask_lt();
+  for (int c = t0; c < t1; c += kBlock) {
+    const int i = c + threadIdx.x;
+    const int b = (i < t1) ? (int)bin_of[i] : -1;
+    int rank = 0;
+    for (int bb = 0; bb < nb; ++bb) {
+      const unsigned long long m = __ballot(b == bb);
+      if (b == bb) rank = __popcll(m & below);
+      if (lane == 0) wcnt[wid][bb] = __popcll(m);
+    }
+    __syncthreads();
+    if (b >= 0) {
+      int pos = run[b] + rank;
+      for (int w = 0; w < wid; ++w) pos += wcnt[w][b];
+      rows_out[pos] = i;
+    }
+    __syncthreads();
+    if (threadIdx.x < nb) {
+      int add = 0;
+      for (int w = 0; w < kBlock / kWave; ++w) add += wcnt[w][threadIdx.x];
+      run[threadIdx.x] += add;
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_partition(int rows, int nbins, PartitionScratch& ps,
+                            int* rows_out, hipStream_t s) {
+  if (rows <= 0) return hipMemsetAsync(ps.bin_offsets, 0,
+                                       (kMaxBins + 1) * sizeof(int), s);
+  hipLaunchKernelGGL(k_part_hist, dim3(ps.ntiles), dim3(kBlock), 0, s, rows,
+                     nbins, ps.tile_rows, ps.bin_of, ps.tile_hist);
+  hipLaunchKernelGGL(k_part_scan, dim3(1), dim3(kBlock), 0, s, ps.ntiles,
+                     nbins, ps.tile_hist, ps.tile_base, ps.bin_offsets);
+  hipLaunchKernelGGL(k_part_scatter, dim3(ps.ntiles), dim3(kBlock), 0, s,
+                     rows, nbins, ps.tile_rows, ps.bin_of, ps.tile_base,
+                     ps.bin_offsets, rows_out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Symbolic count (reference: rvsp_symbolic.c:13-88, distinct columns of the
+// union of B rows).
+
+// LDS open-addressing hash of 2^log2T keys per row, G lanes per row.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_count_hash(
+    const int* __restrict__ rows, int nrows, int log2T, DevCsr A, DevCsr B,
+    int* __restrict__ cnt) {
+  extern __shared__ int lds[];
+  constexpr int kGroups = kBlock / G;
+  const int T = 1 << log2T;
+  const int g = threadIdx.x / G;
+  const int l = threadIdx.x & (G - 1);
+  const int r = blockIdx.x * kGroups + g;
+  if (r >= nrows) return;  // uniform per group; no block barriers below
+  int* keys = lds + g * T;
+  const int row = rows[r];
+  for (int j = l; j < T; j += G) keys[j] = kEmptyKey;
+  int mine = 0;
+  group_for_each_col<G>(A.rp[row], A.rp[row + 1], A.ci, B.rp, B.ci,
+                        [&](int col) {
+                          int slot = hash_slot(col, log2T);
+                          while (true) {
+                            const int old = atomicCAS(&keys[slot], kEmptyKey, col);
+                            if (old == kEmptyKey) { ++mine; break; }
+                            if (old == col) break;
+                            slot = (slot + 1) & (T - 1);
+                          }
+                        });
+  mine = group_sum<G>(mine);
+  if (l == 0) cnt[row] = mine;
+}
+
+// One workgroup per row. The row's columns lie in [lo, hi]; a bitmap over that
+// span lives in LDS (GLOBAL_BM == false, span <= 32 * words) or in this
+// workgroup's slot of global scratch (GLOBAL_BM == true, absolute column
+// bits, words cleared again after use). FILL == false counts set bits;
+// FILL == true writes the set columns in ascending order to C.
+template <bool GLOBAL_BM, bool FILL>
+__global__ __launch_bounds__(kBlock) void k_bitmap_row(
+    const int* __restrict__ rows, int nrows, DevCsr A, DevCsr B, RowStats st,
+    int* __restrict__ cnt, const int* __restrict__ c_rp, int* __restrict__ c_ci,
+    unsigned* __restrict__ gbits, size_t slot_words) {
+  extern __shared__ unsigned lbits[];
+  __shared__ int sh_excl[kBlock], sh_bs[kBlock], tmp[kBlock / kWave];
+  unsigned* bits = GLOBAL_BM ? gbits + (size_t)blockIdx.x * slot_words : lbits;
+  for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
+    const int row = rows[r];
+    const int lo = st.lo[row], hi = st.hi[row];
+    // LDS bitmaps are offset by lo; global ones are absolute.
+    const int w0 = GLOBAL_BM ? (lo >> 5) : 0;
+    const int w1 = GLOBAL_BM ? (hi >> 5) + 1 : ((hi - lo) >> 5) + 1;
+    const int base_col = GLOBAL_BM ? 0 : lo;
+    if (!GLOBAL_BM) {
+      for (int w = threadIdx.x; w < w1; w += kBlock) bits[w] = 0u;
+      __syncthreads();
+    }
+    block_for_each_col(A.rp[row], A.rp[row + 1], A.ci, B.rp, B.ci, sh_excl,
+                       sh_bs, tmp, [&](int col) {
+                         const int d = col - base_col;
+                         __hip_atomic_fetch_or(&bits[d >> 5], 1u << (d & 31),
+                                               __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                       });
+    // block_for_each_col ends with a barrier, which also orders the global
+    // atomics above before the reads below (workgroup scope).
+    const int nw = w1 - w0;
+    const int per = (nw + kBlock - 1) / kBlock;
+    const int my0 = w0 + min(nw, (int)threadIdx.x * per);
+    const int my1 = w0 + min(nw, ((int)threadIdx.x + 1) * per);
+    int c = 0;
+    for (int w = my0; w < my1; ++w) c += __popc(bits[w]);
+    int total;
+    int off = block_excl_scan(c, tmp, &total);
+    if (!FILL) {
+      if (threadIdx.x == 0) cnt[row] = total;
+      if (GLOBAL_BM)
+        for (int w = my0; w < my1; ++w) bits[w] = 0u;
+    } else {
+      int* out = c_ci + c_rp[row];
+      for (int w = my0; w < my1; ++w) {
+        unsigned m = bits[w];
+        if (GLOBAL_BM) bits[w] = 0u;
+        while (m) {
+          const int b = __ffs(m) - 1;
+          out[off++] = base_col + w * 32 + b;
+          m &= m - 1;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int G>
+static hipError_t launch_count_hash_g(const BinSpec& b, const int* rows,
+                                      int nrows, const DevCsr& A,
+                                      const DevCsr& B, int* cnt, hipStream_t s) {
+  const int groups = kBlock / G;
+  const int grid = (nrows + groups - 1) / groups;
+  int log2T = 0;
+  while ((1 << log2T) < b.cap) ++log2T;
+  hipLaunchKernelGGL(k_count_hash<G>, dim3(grid), dim3(kBlock), b.lds_bytes, s,
+                     rows, nrows, log2T, A, B, cnt);
+  return hipGetLastError();
+}
+
+static int row_grid(int nrows) { return nrows < 65535 ? nrows : 65535; }
+
+hipError_t launch_count_bin(const BinSpec& b, const int* rows, int nrows,
+                            const DevCsr& A, const DevCsr& B, RowStats st,
+                            int* cnt, const GlobalSlots& gs, hipStream_t s) {
+  if (nrows <= 0) return hipSuccess;
+  switch (b.kind) {
+    case KIND_HASH:
+      switch (b.group) {
+        case 1: return launch_count_hash_g<1>(b, rows, nrows, A, B, cnt, s);
+        case 2: return launch_count_hash_g<2>(b, rows, nrows, A, B, cnt, s);
+        case 4: return launch_count_hash_g<4>(b, rows, nrows, A, B, cnt, s);
+        case 8: return launch_count_hash_g<8>(b, rows, nrows, A, B, cnt, s);
+        case 16: return launch_count_hash_g<16>(b, rows, nrows, A, B, cnt, s);
+        case 32: return launch_count_hash_g<32>(b, rows, nrows, A, B, cnt, s);
+        case 64: return launch_count_hash_g<64>(b, rows, nrows, A, B, cnt, s);
+        default: return hipErrorInvalidValue;
+      }
+    case KIND_BITMAP:
+      hipLaunchKernelGGL((k_bitmap_row<false, false>), dim3(row_grid(nrows)),
+                         dim3(kBlock), b.lds_bytes, s, rows, nrows, A, B, st,
+                         cnt, nullptr, nullptr, nullptr, (size_t)0);
+      return hipGetLastError();
+    case KIND_GLOBAL:
+      if (!gs.base || gs.nslots <= 0) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((k_bitmap_row<true, false>),
+                         dim3(min(nrows, gs.nslots)), dim3(kBlock), 0, s, rows,
+                         nrows, A, B, st, cnt, nullptr, nullptr,
+                         (unsigned*)gs.base, gs.slot_bytes / sizeof(unsigned));
+      return hipGetLastError();
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of the per-row counts into C.row_ptr (reference:
+// rvsp_symbolic.c:75-85). Sums are kept in 64 bits so the caller can apply
+// the reference's INT32_MAX guard (rvsp_symbolic.c:69-72) to the total.
+
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+size_t scan_tile_count(int rows) {
+  return (size_t)((rows + kScanTile - 1) / kScanTile) + 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(
+    const int* __restrict__ cnt, int n, long long* __restrict__ tile_sums) {
+  __shared__ long long tmp[kBlock / kWave];
+  const int base = blockIdx.x * kScanTile;
+  long long s = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const int i = base + j * kBlock + threadIdx.x;
+    if (i < n) s += cnt[i];
+  }
+  s = block_sum(s, tmp);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_top(
+    long long* __restrict__ tile_sums, int ntiles, long long* __restrict__ total) {
+  __shared__ long long tmp[kBlock / kWave];
+  long long running = 0;
+  for (int t0 = 0; t0 < ntiles; t0 += kBlock) {
+    const int t = t0 + threadIdx.x;
+    const long long v = (t < ntiles) ? tile_sums[t] : 0;
+    long long tot;
+    const long long ex = block_excl_scan(v, tmp, &tot);
+    if (t < ntiles) tile_sums[t] = running + ex;
+    running += tot;
+  }
+  if (threadIdx.x == 0) *total = running;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply(
+    const int* __restrict__ cnt, int n, const long long* __restrict__ tile_off,
+    int* __restrict__ c_rp) {
+  __shared__ long long tmp[kBlock / kWave];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int v[kScanItems];
+  long long s = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    v[j] = (base + j < n) ? cnt[base + j] : 0;
+    s += v[j];
+  }
+  long long tot;
+  long long run = block_excl_scan(s, tmp, &tot) + tile_off[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    if (base + j < n) {
+      c_rp[base + j] = (int)run;
+      run += v[j];
+      if (base + j == n - 1) c_rp[n] = (int)run;
+    }
+  }
+}
+
+hipError_t launch_scan(const int* cnt, int rows, int* c_rp, long long* total,
+                       long long* tile_sums, hipStream_t s) {
+  const int ntiles = (rows + kScanTile - 1) / kScanTile;
+  if (rows <= 0) return hipMemsetAsync(total, 0, sizeof(long long), s);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(kBlock), 0, s, cnt, rows,
+                     tile_sums);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, s, tile_sums, ntiles,
+                     total);
+  hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(kBlock), 0, s, cnt, rows,
+                     tile_sums, c_rp);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Fill: sorted column indices of every C row (reference:
+// rvsp_symbolic.c:91-138 with rvsp_sort.h; only the ascending result is
+// observable, not the sort algorithm).
+
+// One nonempty B row: the C row's columns are that row's columns.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_fill_direct(
+    const int* __restrict__ rows, int nrows, DevCsr A, DevCsr B,
+    const int* __restrict__ c_rp, int* __restrict__ c_ci) {
+  constexpr int kGroups = kBlock / G;
+  const int g = threadIdx.x / G;
+  const int l = threadIdx.x & (G - 1);
+  const int r = blockIdx.x * kGroups + g;
+  if (r >= nrows) return;
+  const int row = rows[r];
+  int* out = c_ci + c_rp[row];
+  for (int ap = A.rp[row]; ap < A.rp[row + 1]; ++ap) {
+    const int k = A.ci[ap];
+    const int bs = B.rp[k], be = B.rp[k + 1];
+    for (int p = bs + l; p < be; p += G) out[p - bs] = B.ci[p];
+  }
+}
+
+// LDS hash of 2^log2T slots plus a compact list of the distinct columns;
+// each column's output position is its rank among the row's columns.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_fill_hash(
+    const int* __restrict__ rows, int nrows, int log2T, DevCsr A, DevCsr B,
+    const int* __restrict__ c_rp, int* __restrict__ c_ci) {
+  extern __shared__ int lds[];
+  constexpr int kGroups = kBlock / G;
+  const int T = 1 << log2T;
+  const int stride = T + T / 2 + 4;
+  const int g = threadIdx.x / G;
+  const int l = threadIdx.x & (G - 1);
+  const int r = blockIdx.x * kGroups + g;
+  if (r >= nrows) return;
+  int* keys = lds + g * stride;
+  int* list = keys + T;
+  int* counter = list + T / 2;
+  const int row = rows[r];
+  for (int j = l; j < T; j += G) keys[j] = kEmptyKey;
+  if (l == 0) *counter = 0;
+  group_for_each_col<G>(A.rp[row], A.rp[row + 1], A.ci, B.rp, B.ci,
+                        [&](int col) {
+                          int slot = hash_slot(col, log2T);
+                          while (true) {
+                            const int old = atomicCAS(&keys[slot], kEmptyKey, col);
+                            if (old == kEmptyKey) {
+                              list[atomicAdd(counter, 1)] = col;
+                              break;
+                            }
+                            if (old == col) break;
+                            slot = (slot + 1) & (T - 1);
+                          }
+                        });
+  const int c_s = c_rp[row];
+  const int n = c_rp[row + 1] - c_s;
+  for (int j = l; j < n; j += G) {
+    const int key = list[j];
+    int rank = 0;
+    for (int q = 0; q < n; ++q) rank += (list[q] < key) ? 1 : 0;
+    c_ci[c_s + rank] = key;
+  }
+}
+
+template <int G>
+static hipError_t launch_fill_g(const BinSpec& b, const int* rows, int nrows,
+                                const DevCsr& A, const DevCsr& B,
+                                const int* c_rp, int* c_ci, hipStream_t s) {
+  const int groups = kBlock / G;
+  const int grid = (nrows + groups - 1) / groups;
+  if (b.kind == KIND_DIRECT) {
+    hipLaunchKernelGGL(k_fill_direct<G>, dim3(grid), dim3(kBlock), 0, s, rows,
+                       nrows, A, B, c_rp, c_ci);
+  } else {
+    int log2T = 0;
+    while ((1 << log2T) < b.cap) ++log2T;
+    hipLaunchKernelGGL(k_fill_hash<G>, dim3(grid), dim3(kBlock), b.lds_bytes, s,
+                       rows, nrows, log2T, A, B, c_rp, c_ci);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_bin(const BinSpec& b, const int* rows, int nrows,
+                           const DevCsr& A, const DevCsr& B, RowStats st,
+                           const int* c_rp, int* c_ci, const GlobalSlots& gs,
+                           hipStream_t s) {
+  if (nrows <= 0) return hipSuccess;
+  switch (b.kind) {
+    case KIND_DIRECT:
+    case KIND_HASH:
+      switch (b.group) {
+        case 1: return launch_fill_g<1>(b, rows, nrows, A, B, c_rp, c_ci, s);
+        case 2: return launch_fill_g<2>(b, rows, nrows, A, B, c_rp, c_ci, s);
+        case 4: return launch_fill_g<4>(b, rows, nrows, A, B, c_rp, c_ci, s);
+        case 8: return launch_fill_g<8>(b, rows, nrows, A, B, c_rp, c_ci, s);
+        case 16: return launch_fill_g<16>(b, rows, nrows, A, B, c_rp, c_ci, s);
+        case 32: return launch_fill_g<32>(b, rows, nrows, A, B, c_rp, c_ci, s);
+        case 64: return launch_fill_g<64>(b, rows, nrows, A, B, c_rp, c_ci, s);
+        default: return hipErrorInvalidValue;
+      }
+    case KIND_BITMAP:
+      hipLaunchKernelGGL((k_bitmap_row<false, true>), dim3(row_grid(nrows)),
+                         dim3(kBlock), b.lds_bytes, s, rows, nrows, A, B, st,
+                         nullptr, c_rp, c_ci, nullptr, (size_t)0);
+      return hipGetLastError();
+    case KIND_GLOBAL:
+      if (!gs.base || gs.nslots <= 0) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((k_bitmap_row<true, true>), dim3(min(nrows, gs.nslots)),
+                         dim3(kBlock), 0, s, rows, nrows, A, B, st, nullptr,
+                         c_rp, c_ci, (unsigned*)gs.base,
+                         gs.slot_bytes / sizeof(unsigned));
+      return hipGetLastError();
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Numeric (reference: gustavson_core_f32.inc:18-64, accum_scalar_f32.c:19-71).
+
+// One nonempty B row: C(i, :) = 0 + a * B(k, :).
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_numeric_direct(
+    const int* __restrict__ rows, int nrows, DevCsr A, DevCsr B,
+    const int* __restrict__ c_rp, float* __restrict__ c_val) {
+  constexpr int kGroups = kBlock / G;
+  const int g = threadIdx.x / G;
+  const int l = threadIdx.x & (G - 1);
+  const int r = blockIdx.x * kGroups + g;
+  if (r >= nrows) return;
+  const int row = rows[r];
+  float* out = c_val + c_rp[row];
+  for (int ap = A.rp[row]; ap < A.rp[row + 1]; ++ap) {
+    const int k = A.ci[ap];
+    const int bs = B.rp[k], be = B.rp[k + 1];
+    const float a = A.val[ap];
+    for (int p = bs + l; p < be; p += G) out[p - bs] = acc_step(0.0f, a, B.val[p]);
+  }
+}
+
+// G lanes per row. The row's sorted C columns sit in LDS next to an FP32
+// accumulator; each product finds its column by binary search. A entries are
+// visited in order and the lanes of the group split the current B row only.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_numeric_search(
+    const int* __restrict__ rows, int nrows, int cap, DevCsr A, DevCsr B,
+    const int* __restrict__ c_rp, const int* __restrict__ c_ci,
+    float* __restrict__ c_val) {
+  extern __shared__ int lds[];
+  constexpr int kGroups = kBlock / G;
+  const int g = threadIdx.x / G;
+  const int l = threadIdx.x & (G - 1);
+  const int r = blockIdx.x * kGroups + g;
+  if (r >= nrows) return;
+  int* cols = lds + g * 2 * cap;
+  float* vals = reinterpret_cast<float*>(cols + cap);
+  const int row = rows[r];
+  const int c_s = c_rp[row];
+  const int n = c_rp[row + 1] - c_s;
+  for (int j = l; j < n; j += G) {
+    cols[j] = c_ci[c_s + j];
+    vals[j] = 0.0f;
+  }
+  const int a_e = A.rp[row + 1];
+  for (int ap = A.rp[row]; ap < a_e; ++ap) {
+    const int k = A.ci[ap];
+    const float a = A.val[ap];
+    const int be = B.rp[k + 1];
+    for (int p = B.rp[k] + l; p < be; p += G) {
+      const int pos = lds_find(cols, n, B.ci[p]);
+      vals[pos] = acc_step(vals[pos], a, B.val[p]);
+    }
+  }
+  for (int j = l; j < n; j += G) c_val[c_s + j] = vals[j];
+}
+
+// First index in the ascending b_ci[s, e) whose value is >= key.
+__device__ __forceinline__ int gm_lower_bound(const int* __restrict__ b_ci, int s,
+                                              int e, int key) {
+  while (s < e) {
+    const int m = (s + e) >> 1;
+    if (b_ci[m] < key) s = m + 1; else e = m;
+  }
+  return s;
+}
+
+// One workgroup per row, each of its four waves owning a contiguous quarter
+// of the row's output positions (hence a contiguous column range). Every wave
+// walks all of A's row in order and applies only the products that land in
+// its own columns, so no two waves ever touch the same C(i,j) and no
+// barrier is needed between A entries. GLOBAL_ACC == false keeps the
+// row's columns and accumulators in LDS (n <= cap); GLOBAL_ACC == true
+// accumulates in this workgroup's dense slot of global memory, indexed by
+// column.
+template <bool GLOBAL_ACC>
+__global__ __launch_bounds__(kBlock) void k_numeric_rowblock(
+    const int* __restrict__ rows, int nrows, int cap, DevCsr A, DevCsr B,
+    const int* __restrict__ c_rp, const int* __restrict__ c_ci,
+    float* __restrict__ c_val, float* __restrict__ gacc, size_t slot_floats) {
+  extern __shared__ int lds[];
+  constexpr int kWaves = kBlock / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x >> 6;
+  int* cols = lds;
+  float* vals = reinterpret_cast<float*>(lds + cap);
+  float* acc = GLOBAL_ACC ? gacc + (size_t)blockIdx.x * slot_floats : nullptr;
+  for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
+    const int row = rows[r];
+    const int c_s = c_rp[row];
+    const int n = c_rp[row + 1] - c_s;
+    // This wave's output positions [p0, p1) and their column range.
+    const int p0 = (int)(((long long)n * wid) / kWaves);
+    const int p1 = (int)(((long long)n * (wid + 1)) / kWaves);
+    if (p1 > p0) {
+      for (int j = p0 + lane; j < p1; j += kWave) {
+        const int col = c_ci[c_s + j];
+        if (GLOBAL_ACC) {
+          acc[col] = 0.0f;
+        } else {
+          cols[j] = col;
+          vals[j] = 0.0f;
+        }
+      }
+      const int clo = c_ci[c_s + p0];
+      const int chi = c_ci[c_s + p1 - 1];
+      const int a_e = A.rp[row + 1];
+      for (int ap = A.rp[row]; ap < a_e; ++ap) {
+        const int k = A.ci[ap];
+        const float a = A.val[ap];
+        int bs = B.rp[k], be = B.rp[k + 1];
+        if (be - bs > 4 * kWave) {
+          // Long B row: narrow it to this wave's columns first.
+          bs = gm_lower_bound(B.ci, bs, be, clo);
+          be = gm_lower_bound(B.ci, bs, be, chi + 1);
+        }
+        for (int p = bs + lane; p < be; p += kWave) {
+          const int col = B.ci[p];
+          if (col < clo || col > chi) continue;
+          if (GLOBAL_ACC) {
+            acc[col] = acc_step(acc[col], a, B.val[p]);
+          } else {
+            const int pos = p0 + lds_find(cols + p0, p1 - p0, col);
+            vals[pos] = acc_step(vals[pos], a, B.val[p]);
+          }
+        }
+      }
+      for (int j = p0 + lane; j < p1; j += kWave)
+        c_val[c_s + j] = GLOBAL_ACC ? acc[c_ci[c_s + j]] : vals[j];
+    }
+    __syncthreads();  // LDS / slot reuse by the next row
+  }
+}
+
+template <int G>
+static hipError_t launch_numeric_g(const BinSpec& b, const int* rows, int nrows,
+                                   const DevCsr& A, const DevCsr& B,
+                                   const int* c_rp, const int* c_ci,
+                                   float* c_val, hipStream_t s) {
+  const int groups = kBlock / G;
+  const int grid = (nrows + groups - 1) / groups;
+  if (b.kind == KIND_DIRECT) {
+    hipLaunchKernelGGL(k_numeric_direct<G>, dim3(grid), dim3(kBlock), 0, s,
+                       rows, nrows, A, B, c_rp, c_val);
+  } else {
+    hipLaunchKernelGGL(k_numeric_search<G>, dim3(grid), dim3(kBlock),
+                       b.lds_bytes, s, rows, nrows, b.cap, A, B, c_rp, c_ci,
+                       c_val);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
+                              const DevCsr& A, const DevCsr& B, RowStats st,
+                              const int* c_rp, const int* c_ci, float* c_val,
+                              const GlobalSlots& gs, hipStream_t s) {
+  (void)st;
+  if (nrows <= 0) return hipSuccess;
+  switch (b.kind) {
+    case KIND_DIRECT:
+    case KIND_SEARCH:
+      switch (b.group) {
+        case 1: return launch_numeric_g<1>(b, rows, nrows, A, B, c_rp, c_ci, c_val, s);
+        case 2: return launch_numeric_g<2>(b, rows, nrows, A, B, c_rp, c_ci, c_val, s);
+        case 4: return launch_numeric_g<4>(b, rows, nrows, A, B, c_rp, c_ci, c_val, s);
+        case 8: return launch_numeric_g<8>(b, rows, nrows, A, B, c_rp, c_ci, c_val, s);
+        case 16: return launch_numeric_g<16>(b, rows, nrows, A, B, c_rp, c_ci, c_val, s);
+        case 32: return launch_numeric_g<32>(b, rows, nrows, A, B, c_rp, c_ci, c_val, s);
+        case 64: return launch_numeric_g<64>(b, rows, nrows, A, B, c_rp, c_ci, c_val, s);
+        default: return hipErrorInvalidValue;
+      }
+    case KIND_BLOCK:
+      hipLaunchKernelGGL(k_numeric_rowblock<false>, dim3(row_grid(nrows)),
+                         dim3(kBlock), b.lds_bytes, s, rows, nrows, b.cap, A, B,
+                         c_rp, c_ci, c_val, nullptr, (size_t)0);
+      return hipGetLastError();
+    case KIND_GLOBAL:
+      if (!gs.base || gs.nslots <= 0) return hipErrorInvalidValue;
+      hipLaunchKernelGGL(k_numeric_rowblock<true>, dim3(min(nrows, gs.nslots)),
+                         dim3(kBlock), 0, s, rows, nrows, 0, A, B, c_rp, c_ci,
+                         c_val, (float*)gs.base, gs.slot_bytes / sizeof(float));
+      return hipGetLastError();
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+// ---------------------------------------------------------------------------
+
+template <typename K>
+static hipError_t allow_lds(K kernel, int bytes) {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+hipError_t init_kernel_attributes(int lds_bytes) {
+  hipError_t e = hipSuccess;
+#define RVSP_ALLOW(k) \
+  if ((e = allow_lds(k, lds_bytes)) != hipSuccess) return e;
+  RVSP_ALLOW(k_count_hash<1>) RVSP_ALLOW(k_count_hash<2>)
+  RVSP_ALLOW(k_count_hash<4>) RVSP_ALLOW(k_count_hash<8>)
+  RVSP_ALLOW(k_count_hash<16>) RVSP_ALLOW(k_count_hash<32>)
+  RVSP_ALLOW(k_count_hash<64>)
+  RVSP_ALLOW(k_fill_hash<1>) RVSP_ALLOW(k_fill_hash<2>)
+  RVSP_ALLOW(k_fill_hash<4>) RVSP_ALLOW(k_fill_hash<8>)
+  RVSP_ALLOW(k_fill_hash<16>) RVSP_ALLOW(k_fill_hash<32>)
+  RVSP_ALLOW(k_fill_hash<64>)
+  RVSP_ALLOW(k_numeric_search<1>) RVSP_ALLOW(k_numeric_search<2>)
+  RVSP_ALLOW(k_numeric_search<4>) RVSP_ALLOW(k_numeric_search<8>)
+  RVSP_ALLOW(k_numeric_search<16>) RVSP_ALLOW(k_numeric_search<32>)
+  RVSP_ALLOW(k_numeric_search<64>)
+  RVSP_ALLOW((k_bitmap_row<false, false>)) RVSP_ALLOW((k_bitmap_row<false, true>))
+  RVSP_ALLOW(k_numeric_rowblock<false>)
+#undef RVSP_ALLOW
+  return e;
+}
+
+}  // namespace rvsp

@@ -1,0 +1,133 @@
+// kernels.hpp — host-side launch interface of the gfx950 SpGEMM kernels.
+//
+// Phases (reference counterparts in parentheses):
+//   analyze    per-row product count Op_i and B-column span
+//              (rvsp_symbolic.c:27-73 op counting; bench.c:215-240)
+//   classify   row -> bin for one phase, from a BinRule table
+//   partition  stable counting sort of rows by bin (ascending row order
+//              inside every bin)
+//   count      exact distinct-column count per row (rvsp_symbolic.c:13-88)
+//   scan       exclusive scan into C.row_ptr with an int64 total
+//              (rvsp_symbolic.c:75-85, overflow guard :69-72)
+//   fill       sorted column indices per row (rvsp_symbolic.c:91-138)
+//   numeric    values in reference accumulation order
+//              (gustavson_core_f32.inc:18-64, accum_scalar_f32.c:19-71)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rvsp {
+
+// Device view of a CSR operand. Pointers are device addresses.
+struct DevCsr {
+  int rows = 0, cols = 0, nnz = 0;
+  const int* rp = nullptr;
+  const int* ci = nullptr;
+  const float* val = nullptr;
+};
+
+// Per-row analysis results (device arrays of length rows(A)).
+struct RowStats {
+  long long* ops = nullptr;  // Op_i = sum of |B_k| over A's row i
+  int* lo = nullptr;         // smallest B column reached by row i (INT_MAX if none)
+  int* hi = nullptr;         // largest B column reached by row i (-1 if none)
+  int* maxb = nullptr;       // longest B row reached by row i
+};
+
+// Accumulator kinds. The same enum names the kernel family used for a bin.
+enum Kind : int {
+  KIND_NONE = 0,     // rows with nothing to do (empty output rows)
+  KIND_DIRECT = 1,   // one A entry: the C row is a scaled copy of one B row
+  KIND_HASH = 2,     // LDS hash table per row, G lanes per row (count, fill)
+  KIND_SEARCH = 3,   // LDS sorted columns + binary search, G lanes per row (numeric)
+  KIND_BITMAP = 4,   // one workgroup per row, LDS bitmap over the row's span (count, fill)
+  KIND_BLOCK = 5,    // one workgroup per row, LDS sorted columns (numeric)
+  KIND_GLOBAL = 6,   // one workgroup per row, dense scratch in global memory
+  KIND_DENSE = 7     // LDS dense accumulator over the row's column span (numeric)
+};
+
+enum Phase : int { PHASE_COUNT = 0, PHASE_FILL = 1, PHASE_NUMERIC = 2 };
+
+constexpr int kMaxBins = 24;
+
+// Row -> bin rule. Bin 0 is reserved for rows that need no table (empty or
+// single-A-entry rows). Other rows go to the first bin b >= 1 whose
+// `bound[b]` >= the row's size metric and whose `span_max[b]` >= the row's
+// column span; rows matching none go to the last bin, which must be a
+// GLOBAL bin. The size metric is the distinct-column upper bound
+// min(Op_i, span_i) for the count phase and the exact nnz of the C row for
+// fill and numeric.
+struct BinRule {
+  int nbins = 0;
+  int bound[kMaxBins];
+  int span_max[kMaxBins];
+};
+
+// Launch parameters of one bin.
+struct BinSpec {
+  int kind = KIND_NONE;
+  int group = 1;     // lanes per row (power of two <= 64, or 256 for block kinds)
+  int cap = 0;       // per-row LDS capacity: table slots (HASH), columns
+                     // (SEARCH/BLOCK), bitmap bits (BITMAP), span (DENSE)
+  int lds_bytes = 0; // dynamic LDS per workgroup
+};
+
+// Scratch for the partition step.
+struct PartitionScratch {
+  unsigned char* bin_of = nullptr;  // rows
+  int* tile_hist = nullptr;         // ntiles * kMaxBins
+  int* tile_base = nullptr;         // ntiles * kMaxBins
+  int* bin_offsets = nullptr;       // kMaxBins + 1
+  int tile_rows = 0;
+  int ntiles = 0;
+};
+
+// Global-memory scratch for GLOBAL bins (one slot per resident workgroup).
+struct GlobalSlots {
+  void* base = nullptr;
+  size_t slot_bytes = 0;
+  int nslots = 0;
+};
+
+// --- analysis ---------------------------------------------------------------
+hipError_t launch_analyze(const DevCsr& A, const DevCsr& B, RowStats st,
+                          int* long_list, int* n_long, hipStream_t s);
+
+// --- binning ----------------------------------------------------------------
+// Count phase: classifies by min(Op_i, span_i); writes cnt[i] directly for
+// rows whose count needs no table (Op_i == 0, one A entry, Op_i == 1).
+hipError_t launch_classify_count(const DevCsr& A, RowStats st, const BinRule& r,
+                                 unsigned char* bin_of, int* cnt, hipStream_t s);
+// Fill/numeric phases: classifies by the exact C-row nnz.
+hipError_t launch_classify_nnz(const DevCsr& A, RowStats st, const int* c_rp,
+                               const BinRule& r, unsigned char* bin_of,
+                               hipStream_t s);
+hipError_t launch_partition(int rows, int nbins, PartitionScratch& ps,
+                            int* rows_out, hipStream_t s);
+
+// --- symbolic -----------------------------------------------------------
+hipError_t launch_count_bin(const BinSpec& b, const int* rows, int nrows,
+                            const DevCsr& A, const DevCsr& B, RowStats st,
+                            int* cnt, const GlobalSlots& gs, hipStream_t s);
+hipError_t launch_scan(const int* cnt, int rows, int* c_rp, long long* total,
+                       long long* tile_sums, hipStream_t s);
+size_t scan_tile_count(int rows);
+hipError_t launch_fill_bin(const BinSpec& b, const int* rows, int nrows,
+                           const DevCsr& A, const DevCsr& B, RowStats st,
+                           const int* c_rp, int* c_ci, const GlobalSlots& gs,
+                           hipStream_t s);
+
+// --- numeric ------------------------------------------------------------
+hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
+                              const DevCsr& A, const DevCsr& B, RowStats st,
+                              const int* c_rp, const int* c_ci, float* c_val,
+                              const GlobalSlots& gs, hipStream_t s);
+
+// One-time kernel attribute setup: allows `lds_bytes` of dynamic LDS.
+hipError_t init_kernel_attributes(int lds_bytes);
+
+// Largest dynamic LDS a workgroup may request on gfx950.
+constexpr int kMaxLdsBytes = 160 * 1024;
+
+}  // namespace rvsp

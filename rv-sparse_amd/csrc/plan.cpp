@@ -1,0 +1,188 @@
+// plan.cpp — the fixed per-bin strategy table and the rules that keep any
+// override launchable.
+#include "plan.hpp"
+
+#include "common.hpp"
+
+#include <climits>
+
+namespace rvsp {
+
+namespace {
+
+int g_lds_limit = kMaxLdsBytes - 8 * 1024;  // dynamic part, set per device
+
+bool pow2(int x) { return x > 0 && (x & (x - 1)) == 0; }
+
+void set_bin(PhaseTable& t, int phase, int b, int kind, int group, int cap,
+             int bound, int span_max) {
+  t.rule.bound[b] = bound;
+  t.rule.span_max[b] = span_max;
+  t.spec[b].kind = kind;
+  t.spec[b].group = group;
+  t.spec[b].cap = cap;
+  t.spec[b].lds_bytes = lds_bytes_for(phase, kind, group, cap);
+}
+
+}  // namespace
+
+void set_lds_limit(int bytes) { g_lds_limit = bytes; }
+int lds_limit() { return g_lds_limit; }
+
+const char* kind_name(int kind) {
+  switch (kind) {
+    case KIND_NONE: return "none";
+    case KIND_DIRECT: return "direct";
+    case KIND_HASH: return "lds_hash";
+    case KIND_SEARCH: return "lds_search";
+    case KIND_BITMAP: return "lds_bitmap";
+    case KIND_BLOCK: return "lds_block";
+    case KIND_GLOBAL: return "global_dense";
+    case KIND_DENSE: return "lds_dense";
+    default: return "?";
+  }
+}
+
+int lds_bytes_for(int phase, int kind, int group, int cap) {
+  const bool sub_wave = pow2(group) && group <= kWave;
+  long long bytes = -1;
+  switch (kind) {
+    case KIND_NONE:
+    case KIND_GLOBAL:
+      bytes = 0;
+      break;
+    case KIND_DIRECT:
+      bytes = sub_wave ? 0 : -1;
+      break;
+    case KIND_HASH:
+      if (!sub_wave || !pow2(cap) || cap < 2 || cap > 8192) break;
+      if (phase == PHASE_COUNT)
+        bytes = (long long)(kBlock / group) * cap * 4;
+      else if (phase == PHASE_FILL)
+        bytes = (long long)(kBlock / group) * (cap + cap / 2 + 4) * 4;
+      break;
+    case KIND_SEARCH:
+      if (phase != PHASE_NUMERIC || !sub_wave || cap < 1) break;
+      bytes = (long long)(kBlock / group) * cap * 8;
+      break;
+    case KIND_BITMAP:
+      if (phase == PHASE_NUMERIC || cap < 32 || (cap & 31)) break;
+      bytes = cap / 8;
+      break;
+    case KIND_BLOCK:
+      if (phase != PHASE_NUMERIC || cap < 1) break;
+      bytes = (long long)cap * 8;
+      break;
+    default:
+      break;
+  }
+  if (bytes > kMaxLdsBytes - 8 * 1024) return -1;
+  return (int)bytes;
+}
+
+Strategy fixed_strategy() {
+  Strategy s;
+  // ---- count: metric = min(Op_i, span_i), an upper bound of the row's nnz.
+  {
+    PhaseTable& t = s.count;
+    int b = 0;
+    set_bin(t, PHASE_COUNT, b++, KIND_NONE, 1, 0, 0, INT_MAX);
+    set_bin(t, PHASE_COUNT, b++, KIND_DIRECT, 16, 0, 0, INT_MAX);
+    const int T[] = {32, 64, 128, 256, 512, 1024, 2048, 4096};
+    const int G[] = {4, 8, 16, 32, 64, 64, 64, 64};
+    for (int i = 0; i < 8; ++i)
+      set_bin(t, PHASE_COUNT, b++, KIND_HASH, G[i], T[i], T[i] / 2, INT_MAX);
+    const int span[] = {1 << 15, 1 << 18, 1 << 20};
+    for (int i = 0; i < 3; ++i)
+      set_bin(t, PHASE_COUNT, b++, KIND_BITMAP, kBlock, span[i], INT_MAX, span[i]);
+    set_bin(t, PHASE_COUNT, b++, KIND_GLOBAL, kBlock, 0, INT_MAX, INT_MAX);
+    t.rule.nbins = b;
+  }
+  // ---- fill: metric = exact nnz of the C row.
+  {
+    PhaseTable& t = s.fill;
+    int b = 0;
+    set_bin(t, PHASE_FILL, b++, KIND_NONE, 1, 0, 0, INT_MAX);
+    set_bin(t, PHASE_FILL, b++, KIND_DIRECT, 16, 0, 0, INT_MAX);
+    const int T[] = {32, 64, 128, 256, 512};
+    const int G[] = {4, 8, 16, 32, 64};
+    for (int i = 0; i < 5; ++i)
+      set_bin(t, PHASE_FILL, b++, KIND_HASH, G[i], T[i], T[i] / 2, INT_MAX);
+    const int span[] = {1 << 15, 1 << 18, 1 << 20};
+    for (int i = 0; i < 3; ++i)
+      set_bin(t, PHASE_FILL, b++, KIND_BITMAP, kBlock, span[i], INT_MAX, span[i]);
+    set_bin(t, PHASE_FILL, b++, KIND_GLOBAL, kBlock, 0, INT_MAX, INT_MAX);
+    t.rule.nbins = b;
+  }
+  // ---- numeric: metric = exact nnz of the C row.
+  {
+    PhaseTable& t = s.numeric;
+    int b = 0;
+    set_bin(t, PHASE_NUMERIC, b++, KIND_NONE, 1, 0, 0, INT_MAX);
+    set_bin(t, PHASE_NUMERIC, b++, KIND_DIRECT, 16, 0, 0, INT_MAX);
+    const int C[] = {16, 32, 64, 128, 256, 512, 1024, 2048};
+    const int G[] = {4, 8, 16, 32, 64, 64, 64, 64};
+    for (int i = 0; i < 8; ++i)
+      set_bin(t, PHASE_NUMERIC, b++, KIND_SEARCH, G[i], C[i], C[i], INT_MAX);
+    set_bin(t, PHASE_NUMERIC, b++, KIND_BLOCK, kBlock, 4096, 4096, INT_MAX);
+    set_bin(t, PHASE_NUMERIC, b++, KIND_BLOCK, kBlock, 16384, 16384, INT_MAX);
+    set_bin(t, PHASE_NUMERIC, b++, KIND_GLOBAL, kBlock, 0, INT_MAX, INT_MAX);
+    t.rule.nbins = b;
+  }
+  return s;
+}
+
+bool apply_choice(PhaseTable& t, int phase, int b, const BinChoice& c,
+                  int max_metric) {
+  BinSpec cand = t.spec[b];
+  if (c.kind) cand.kind = c.kind;
+  if (c.group) cand.group = c.group;
+  if (c.cap) cand.cap = c.cap;
+  bool ok = true;
+  if (b == 1) {
+    // The direct bin needs no table; only its lane count may change.
+    ok = cand.kind == KIND_DIRECT;
+  } else {
+    switch (cand.kind) {
+      case KIND_HASH:
+        ok = phase != PHASE_NUMERIC && max_metric <= cand.cap / 2;
+        break;
+      case KIND_SEARCH:
+        ok = phase == PHASE_NUMERIC && max_metric <= cand.cap;
+        break;
+      case KIND_BLOCK:
+        ok = phase == PHASE_NUMERIC && max_metric <= cand.cap;
+        if (ok) cand.group = kBlock;
+        break;
+      case KIND_BITMAP:
+        // Bitmap bins are selected by span in the rule, so the kind may not
+        // be moved onto a bin whose rows were not span-checked.
+        ok = t.spec[b].kind == KIND_BITMAP && cand.cap >= t.rule.span_max[b];
+        if (ok) cand.group = kBlock;
+        break;
+      case KIND_GLOBAL:
+        cand.group = kBlock;
+        cand.cap = 0;
+        break;
+      default:
+        ok = false;
+    }
+  }
+  if (ok) {
+    cand.lds_bytes = lds_bytes_for(phase, cand.kind, cand.group, cand.cap);
+    ok = cand.lds_bytes >= 0;
+  }
+  if (ok) {
+    t.spec[b] = cand;
+    return true;
+  }
+  if (b != 1) {
+    t.spec[b].kind = KIND_GLOBAL;
+    t.spec[b].group = kBlock;
+    t.spec[b].cap = 0;
+    t.spec[b].lds_bytes = 0;
+  }
+  return false;
+}
+
+}  // namespace rvsp

@@ -1,0 +1,24 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (os.path.join(ROOT, "rv-sparse_amd"), ROOT, os.path.dirname(os.path.abspath(__file__))):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+    config.addinivalue_line("markers", "slow: full-size configuration, minutes on CPU")
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    """Fails (not skips) when a GPU test runs without a usable device: a GPU
+    test must exercise the HIP library, never pass silently."""
+    import rvsp
+    n = rvsp.device_count()
+    assert n >= 1, "no usable gfx950 device; the HIP library cannot run"
+    return n
