@@ -122,9 +122,14 @@ __device__ __forceinline__ T block_sum(T v, T* tmp) {
 
 // Exact FP32 accumulate step of the reference scalar kernel
 // (accum_scalar_f32.c:67-70 compiled under -std=c11, i.e. without FMA
-// contraction): the product is rounded, then the sum is rounded.
+// contraction): the product is rounded, then the sum is rounded. In this
+// toolchain __fmul_rn/__fadd_rn are plain operators that hipcc would fuse
+// into v_fma_f32 under its default -ffp-contract=fast, so contraction is
+// switched off here explicitly (and for the whole library in the Makefile).
 __device__ __forceinline__ float acc_step(float acc, float a, float b) {
-  return __fadd_rn(acc, __fmul_rn(a, b));
+#pragma clang fp contract(off)
+  const float prod = a * b;
+  return acc + prod;
 }
 
 }  // namespace rvsp
