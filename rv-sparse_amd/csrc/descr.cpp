@@ -211,13 +211,14 @@ int bin_rows(const PhasePlan& p, int b) { return p.offsets[b + 1] - p.offsets[b]
 
 // Sizes the global scratch slots a phase's GLOBAL bins need. Bitmaps are
 // cleared once at allocation; the kernels leave them cleared.
-rvsp_status_t ensure_slots(rvsp_spgemm_descr* d, const PhasePlan& p, bool dense,
-                           hipStream_t s) {
+rvsp_status_t ensure_slots(rvsp_spgemm_descr* d, const PhasePlan& p, int b_cols,
+                           bool dense, hipStream_t s) {
   int rows = 0;
   for (int b = 2; b < p.table.rule.nbins; ++b)
     if (p.table.spec[b].kind == KIND_GLOBAL) rows += bin_rows(p, b);
   if (rows == 0) return RVSP_SUCCESS;
-  const size_t n = (size_t)d->b_cols;
+  if (b_cols <= 0) return RVSP_ERROR_INVALID_ARGUMENT;
+  const size_t n = (size_t)b_cols;
   const size_t slot = dense ? ((n * 4 + 255) & ~(size_t)255)
                             : ((((n + 31) / 32) * 4 + 255) & ~(size_t)255);
   const size_t budget = (size_t)16 << 30;
@@ -308,7 +309,7 @@ rvsp_status_t estimate(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
   RVSP_TRY(partition_phase(d, d->count, M, 0, s));
   PhasePlan* cp[] = {&d->count};
   RVSP_TRY(fetch_offsets(d, cp, 1, s));
-  RVSP_TRY(ensure_slots(d, d->count, false, s));
+  RVSP_TRY(ensure_slots(d, d->count, B.cols, false, s));
   for (int b = 2; b < d->count.table.rule.nbins; ++b) {
     const int n = bin_rows(d->count, b);
     if (n == 0) continue;
@@ -345,8 +346,8 @@ rvsp_status_t estimate(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
                                   d->numeric.table.rule.nbins + 1);
   }
   if (d->algo == RVSP_SPGEMM_ALGO_HIP_LEARNED) RVSP_TRY(apply_learned(d, A, B, s));
-  RVSP_TRY(ensure_slots(d, d->fill, false, s));
-  RVSP_TRY(ensure_slots(d, d->numeric, true, s));
+  RVSP_TRY(ensure_slots(d, d->fill, B.cols, false, s));
+  RVSP_TRY(ensure_slots(d, d->numeric, B.cols, true, s));
   return RVSP_SUCCESS;
 }
 
