@@ -112,6 +112,29 @@ rvsp_status_t rvsp_spgemm_set_bin_choices(rvsp_spgemm_descr_t descr,
                                           const rvsp_bin_choice_t *choices,
                                           int32_t n);
 
+/*
+ * Autotuner. Does everything rvsp_spgemm_compute does (same arguments and
+ * checks) and, in addition, times every launchable kernel shape of every
+ * non-empty bin of the descriptor's plan on these operands (`reps` timed
+ * launches each), keeps the fastest in the descriptor, and records it in the
+ * process-wide tuning table under (phase, bin, mean B-row length class).
+ * Descriptors with RVSP_SPGEMM_ALGO_HIP_TUNED look their bins up in that
+ * table at work_estimation. C holds the product on return.
+ */
+rvsp_status_t rvsp_spgemm_autotune(rvsp_spgemm_descr_t descr,
+                                   const rvsp_csr_matrix_t *A,
+                                   const rvsp_csr_matrix_t *B,
+                                   rvsp_csr_matrix_t *C, void *workspace,
+                                   int32_t reps);
+
+/* Tuning table persistence. path NULL means $RVSP_TUNING_TABLE, else
+ * tuning_table.txt next to the library (also read automatically on the first
+ * HIP_TUNED estimation). save/load return the entry count or -1. */
+int32_t rvsp_tuning_table_save(const char *path);
+int32_t rvsp_tuning_table_load(const char *path);
+void rvsp_tuning_table_clear(void);
+int32_t rvsp_tuning_table_size(void);
+
 /* Device memory helpers for callers without a HIP binding of their own
  * (for example ctypes). kind: 1 host->device, 2 device->host,
  * 3 device->device. */

@@ -18,6 +18,7 @@
 #include "rv_sparse.h"
 #include "rv_sparse_hip.h"
 #include "rvsp_spgemm.h"
+#include "selector.hpp"
 
 using namespace rvsp;
 
@@ -194,10 +195,11 @@ rvsp_status_t partition_phase(rvsp_spgemm_descr* d, PhasePlan& plan, int M,
 }
 
 rvsp_status_t fetch_offsets(rvsp_spgemm_descr* d, PhasePlan* plans[], int n,
-                            hipStream_t s) {
+                            int first_slot, hipStream_t s) {
   std::vector<int> host((size_t)n * (kMaxBins + 1));
-  RVSP_HIP_TRY(hipMemcpyAsync(host.data(), d->bin_offsets.p, host.size() * 4,
-                              hipMemcpyDeviceToHost, s));
+  RVSP_HIP_TRY(hipMemcpyAsync(host.data(),
+                              (const int*)d->bin_offsets.p + first_slot * (kMaxBins + 1),
+                              host.size() * 4, hipMemcpyDeviceToHost, s));
   RVSP_HIP_TRY(hipStreamSynchronize(s));
   for (int i = 0; i < n; ++i) {
     const int nb = plans[i]->table.rule.nbins;
@@ -238,12 +240,6 @@ rvsp_status_t ensure_slots(rvsp_spgemm_descr* d, const PhasePlan& p, int b_cols,
   return RVSP_SUCCESS;
 }
 
-Strategy strategy_for(rvsp_spgemm_descr* d) {
-  Strategy st = fixed_strategy();
-  if (d->algo == RVSP_SPGEMM_ALGO_HIP_TUNED) apply_tuned_table(st, d);
-  return st;
-}
-
 // Bins whose LDS the device cannot grant run as GLOBAL bins.
 void clamp_to_device(PhaseTable& t) {
   for (int b = 2; b < t.rule.nbins; ++b)
@@ -255,15 +251,30 @@ void clamp_to_device(PhaseTable& t) {
     }
 }
 
-void apply_choices(rvsp_spgemm_descr* d, PhaseTable& t, int phase) {
+// Per-bin shape selection of one partitioned phase: the tuned table or the
+// learned selector (by algo), then the caller's explicit overrides, then the
+// device's LDS limit. Every step goes through apply_choice, which vetoes a
+// shape that cannot hold the bin's rows.
+rvsp_status_t select_phase(rvsp_spgemm_descr* d, PhasePlan& p, int phase, const DevCsr& A,
+                           const int* c_rp, hipStream_t s) {
+  const bool tuned = d->algo == RVSP_SPGEMM_ALGO_HIP_TUNED;
+  const bool learned = d->algo == RVSP_SPGEMM_ALGO_HIP_LEARNED;
+  if (tuned || learned) {
+    std::vector<BinFeatures> f;
+    RVSP_TRY(bin_features(d, p, phase, A.rp, c_rp, s, &f));
+    if (tuned) select_tuned(p.table, phase, f);
+    if (learned) RVSP_TRY(select_learned(p.table, phase, f, s));
+  }
   for (const rvsp_bin_choice_t& c : d->choices) {
-    if (c.phase != phase || c.bin < 1 || c.bin >= t.rule.nbins) continue;
+    if (c.phase != phase || c.bin < 1 || c.bin >= p.table.rule.nbins) continue;
     BinChoice bc;
     bc.kind = c.kind;
     bc.group = c.group;
     bc.cap = c.cap;
-    apply_choice(t, phase, c.bin, bc, t.rule.bound[c.bin]);
+    apply_choice(p.table, phase, c.bin, bc, p.table.rule.bound[c.bin]);
   }
+  clamp_to_device(p.table);
+  return RVSP_SUCCESS;
 }
 
 // The symbolic phase on the device: analysis, binning, exact count, scan,
@@ -292,23 +303,18 @@ rvsp_status_t estimate(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
   const RowStats st = row_stats(d);
   RVSP_HIP_TRY(launch_analyze(A, B, st, (int*)d->long_list.p, (int*)d->n_long.p, s));
 
-  Strategy strat = strategy_for(d);
+  const Strategy strat = fixed_strategy();
   d->count.table = strat.count;
   d->fill.table = strat.fill;
   d->numeric.table = strat.numeric;
-  apply_choices(d, d->count.table, PHASE_COUNT);
-  apply_choices(d, d->fill.table, PHASE_FILL);
-  apply_choices(d, d->numeric.table, PHASE_NUMERIC);
-  clamp_to_device(d->count.table);
-  clamp_to_device(d->fill.table);
-  clamp_to_device(d->numeric.table);
 
   // Count.
   RVSP_HIP_TRY(launch_classify_count(A, st, d->count.table.rule,
                                      (unsigned char*)d->bin_of.p, (int*)d->cnt.p, s));
   RVSP_TRY(partition_phase(d, d->count, M, 0, s));
   PhasePlan* cp[] = {&d->count};
-  RVSP_TRY(fetch_offsets(d, cp, 1, s));
+  RVSP_TRY(fetch_offsets(d, cp, 1, 0, s));
+  RVSP_TRY(select_phase(d, d->count, PHASE_COUNT, A, nullptr, s));
   RVSP_TRY(ensure_slots(d, d->count, B.cols, false, s));
   for (int b = 2; b < d->count.table.rule.nbins; ++b) {
     const int n = bin_rows(d->count, b);
@@ -333,19 +339,10 @@ rvsp_status_t estimate(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
                                    d->numeric.table.rule,
                                    (unsigned char*)d->bin_of.p, s));
   RVSP_TRY(partition_phase(d, d->numeric, M, 2, s));
-  // Offsets of both plans sit at slots 1 and 2.
-  {
-    std::vector<int> host(3 * (kMaxBins + 1));
-    RVSP_HIP_TRY(hipMemcpyAsync(host.data(), d->bin_offsets.p, host.size() * 4,
-                                hipMemcpyDeviceToHost, s));
-    RVSP_HIP_TRY(hipStreamSynchronize(s));
-    d->fill.offsets.assign(host.begin() + (kMaxBins + 1),
-                           host.begin() + (kMaxBins + 1) + d->fill.table.rule.nbins + 1);
-    d->numeric.offsets.assign(host.begin() + 2 * (kMaxBins + 1),
-                              host.begin() + 2 * (kMaxBins + 1) +
-                                  d->numeric.table.rule.nbins + 1);
-  }
-  if (d->algo == RVSP_SPGEMM_ALGO_HIP_LEARNED) RVSP_TRY(apply_learned(d, A, B, s));
+  PhasePlan* fp[] = {&d->fill, &d->numeric};
+  RVSP_TRY(fetch_offsets(d, fp, 2, 1, s));
+  RVSP_TRY(select_phase(d, d->fill, PHASE_FILL, A, (const int*)d->c_rp.p, s));
+  RVSP_TRY(select_phase(d, d->numeric, PHASE_NUMERIC, A, (const int*)d->c_rp.p, s));
   RVSP_TRY(ensure_slots(d, d->fill, B.cols, false, s));
   RVSP_TRY(ensure_slots(d, d->numeric, B.cols, true, s));
   return RVSP_SUCCESS;
@@ -407,7 +404,7 @@ void rvsp_spgemm_descr_destroy(rvsp_spgemm_descr_t descr) {
                     &d->cnt, &d->c_rp, &d->scan_tiles, &d->total, &d->bin_of,
                     &d->tile_hist, &d->tile_base, &d->bin_offsets, &d->count.rows,
                     &d->fill.rows, &d->numeric.rows, &d->c_ci_dev, &d->c_val_dev,
-                    &d->gbits, &d->gdense, &d->learn_feat};
+                    &d->gbits, &d->gdense, &d->learn_feat, &d->bin_stats};
   for (DevBuf* b : bufs) release(*b);
   destroy_graph(d);
   if (d->own_stream) (void)hipStreamDestroy(d->own_stream);
@@ -772,6 +769,33 @@ rvsp_status_t rvsp_spgemm_set_bin_choices(rvsp_spgemm_descr_t descr,
   descr->choices.assign(choices, choices + n);
   return RVSP_SUCCESS;
 }
+
+// Autotuner: compute-equivalent call that additionally times every shape of
+// every non-empty bin and keeps the fastest (see selector.cpp).
+rvsp_status_t rvsp_spgemm_autotune(rvsp_spgemm_descr_t descr, const rvsp_csr_matrix_t* A,
+                                   const rvsp_csr_matrix_t* B, rvsp_csr_matrix_t* C,
+                                   void* workspace, int32_t reps) {
+  RVSP_TRY(rvsp_spgemm_compute(descr, A, B, C, workspace));
+  rvsp_spgemm_descr* d = descr;
+  hipStream_t s;
+  RVSP_TRY(descr_stream(d, &s));
+  DevCsr dA, dB;
+  RVSP_TRY(operand_views(d, A, B, false, false, &dA, &dB, s));
+  int* c_ci = d->memory == RVSP_MEMORY_DEVICE ? C->col_idx : (int*)d->c_ci_dev.p;
+  float* c_val = d->memory == RVSP_MEMORY_DEVICE ? (float*)C->values : (float*)d->c_val_dev.p;
+  RVSP_TRY(autotune_plan(d, dA, dB, c_ci, c_val, reps, s));
+  if (d->memory == RVSP_MEMORY_HOST && d->c_nnz > 0) {
+    RVSP_HIP_TRY(hipMemcpyAsync(C->values, d->c_val_dev.p, (size_t)d->c_nnz * 4,
+                                hipMemcpyDeviceToHost, s));
+    RVSP_HIP_TRY(hipStreamSynchronize(s));
+  }
+  return RVSP_SUCCESS;
+}
+
+int32_t rvsp_tuning_table_save(const char* path) { return tuning_table_save(path); }
+int32_t rvsp_tuning_table_load(const char* path) { return tuning_table_load(path); }
+void rvsp_tuning_table_clear(void) { tuning_table_clear(); }
+int32_t rvsp_tuning_table_size(void) { return tuning_table_size(); }
 
 rvsp_status_t rvsp_hip_malloc(void** ptr, size_t bytes) {
   if (ptr == NULL) return RVSP_ERROR_NULL_POINTER;

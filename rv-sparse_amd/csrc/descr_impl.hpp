@@ -72,8 +72,8 @@ struct rvsp_spgemm_descr {
   rvsp::DevBuf gbits, gdense;
   rvsp::GlobalSlots gs_bits, gs_dense;
 
-  // Learned selector features (per numeric bin).
-  rvsp::DevBuf learn_feat;
+  // Per-bin feature scratch (tuned and learned selectors).
+  rvsp::DevBuf learn_feat, bin_stats;
 
   // Captured numeric-only graph.
   hipGraph_t graph = nullptr;
@@ -90,12 +90,6 @@ rvsp_status_t run_fill(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
 rvsp_status_t run_numeric(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
                           const int* c_ci, float* c_val, hipStream_t s);
 
-// Autotuned per-bin table (RVSP_SPGEMM_ALGO_HIP_TUNED).
-void apply_tuned_table(Strategy& st, rvsp_spgemm_descr* d);
-// Learned per-bin accumulator choice (RVSP_SPGEMM_ALGO_HIP_LEARNED), applied
-// to the fill and numeric plans after binning.
-rvsp_status_t apply_learned(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
-                            hipStream_t s);
 void destroy_graph(rvsp_spgemm_descr* d);
 
 }  // namespace rvsp

@@ -358,6 +358,64 @@ hipError_t launch_partition(int rows, int nbins, PartitionScratch& ps,
 }
 
 // ---------------------------------------------------------------------------
+// Per-bin statistics of a partitioned phase (features of the tuned and learned
+// selectors). out[b * kBinStats + s], s = rows, sum nnz(A row), sum Op_i,
+// sum size metric, max size metric, sum span, max longest-B-row, max Op_i.
+// The size metric is min(Op_i, span) when c_rp is null (count phase), else
+// the C row's nnz.
+
+__global__ __launch_bounds__(kBlock) void k_bin_stats(
+    const int* __restrict__ rows, const int* __restrict__ offsets, int nbins,
+    const int* __restrict__ a_rp, RowStats st, const int* __restrict__ c_rp,
+    unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long acc[kMaxBins * kBinStats];
+  __shared__ int off[kMaxBins + 1];
+  for (int i = threadIdx.x; i < kMaxBins * kBinStats; i += kBlock) acc[i] = 0;
+  if (threadIdx.x <= nbins) off[threadIdx.x] = offsets[threadIdx.x];
+  __syncthreads();
+  const int total = off[nbins];
+  for (int p = blockIdx.x * kBlock + threadIdx.x; p < total; p += gridDim.x * kBlock) {
+    int b = 0;
+    while (b + 1 < nbins && off[b + 1] <= p) ++b;
+    const int row = rows[p];
+    const long long ops = st.ops[row];
+    const int span = ops > 0 ? st.hi[row] - st.lo[row] + 1 : 0;
+    const long long metric = c_rp ? (long long)(c_rp[row + 1] - c_rp[row])
+                                  : min(ops, (long long)span);
+    unsigned long long* a = acc + b * kBinStats;
+    atomicAdd(&a[0], 1ull);
+    atomicAdd(&a[1], (unsigned long long)(a_rp[row + 1] - a_rp[row]));
+    atomicAdd(&a[2], (unsigned long long)ops);
+    atomicAdd(&a[3], (unsigned long long)metric);
+    atomicMax(&a[4], (unsigned long long)metric);
+    atomicAdd(&a[5], (unsigned long long)span);
+    atomicMax(&a[6], (unsigned long long)st.maxb[row]);
+    atomicMax(&a[7], (unsigned long long)ops);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nbins * kBinStats; i += kBlock) {
+    if (acc[i] == 0) continue;
+    const int s = i % kBinStats;
+    if (s == 4 || s == 6 || s == 7)
+      atomicMax(&out[i], acc[i]);
+    else
+      atomicAdd(&out[i], acc[i]);
+  }
+}
+
+hipError_t launch_bin_stats(const int* rows, const int* offsets, int nbins, int total_rows,
+                            const int* a_rp, RowStats st, const int* c_rp,
+                            unsigned long long* out, hipStream_t s) {
+  hipError_t e = hipMemsetAsync(out, 0, sizeof(unsigned long long) * kMaxBins * kBinStats, s);
+  if (e != hipSuccess || total_rows <= 0) return e;
+  int grid = (total_rows + kBlock * 8 - 1) / (kBlock * 8);
+  grid = grid < 1024 ? grid : 1024;
+  hipLaunchKernelGGL(k_bin_stats, dim3(grid), dim3(kBlock), 0, s, rows, offsets, nbins,
+                     a_rp, st, c_rp, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Symbolic count (reference: rvsp_symbolic.c:13-88, distinct columns of the
 // union of B rows).
 
@@ -721,9 +779,25 @@ __global__ __launch_bounds__(kBlock) void k_numeric_direct(
   }
 }
 
+// Block b is dispatched to XCD b % 8, and each XCD has its own L2. Renumber
+// blocks so every XCD works through one contiguous range of logical blocks:
+// neighbouring rows, which read overlapping B rows, then share an L2.
+// A bijection on [0, n) for any n.
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
 // G lanes per row. The row's sorted C columns sit in LDS next to an FP32
 // accumulator; each product finds its column by binary search. A entries are
 // visited in order and the lanes of the group split the current B row only.
+//
+// When the workgroup's rows are consecutive (always for a bin holding a
+// banded or uniform matrix, whose rows stay in ascending order), its C rows
+// form one contiguous segment: the workgroup then loads the columns and
+// stores the values of the whole segment with coalesced accesses, and each
+// group works on its row's slice of the segment in LDS. Otherwise every
+// group stages its own row in a fixed slot of `cap` entries.
 template <int G>
 __global__ __launch_bounds__(kBlock) void k_numeric_search(
     const int* __restrict__ rows, int nrows, int cap, DevCsr A, DevCsr B,
@@ -731,30 +805,57 @@ __global__ __launch_bounds__(kBlock) void k_numeric_search(
     float* __restrict__ c_val) {
   extern __shared__ int lds[];
   constexpr int kGroups = kBlock / G;
+  int* cols = lds;
+  float* vals = reinterpret_cast<float*>(lds + kGroups * cap);
+  const int r0 = xcd_block(blockIdx.x, gridDim.x) * kGroups;
+  const int nr = min(kGroups, nrows - r0);
   const int g = threadIdx.x / G;
   const int l = threadIdx.x & (G - 1);
-  const int r = blockIdx.x * kGroups + g;
-  if (r >= nrows) return;
-  int* cols = lds + g * 2 * cap;
-  float* vals = reinterpret_cast<float*>(cols + cap);
-  const int row = rows[r];
+  const bool active = g < nr;
+  const int first = rows[r0];
+  const int last = rows[r0 + nr - 1];
+  const bool seg = (last - first == nr - 1);  // uniform over the block
+  const int row = active ? rows[r0 + g] : first;
   const int c_s = c_rp[row];
-  const int n = c_rp[row + 1] - c_s;
-  for (int j = l; j < n; j += G) {
-    cols[j] = c_ci[c_s + j];
-    vals[j] = 0.0f;
-  }
-  const int a_e = A.rp[row + 1];
-  for (int ap = A.rp[row]; ap < a_e; ++ap) {
-    const int k = A.ci[ap];
-    const float a = A.val[ap];
-    const int be = B.rp[k + 1];
-    for (int p = B.rp[k] + l; p < be; p += G) {
-      const int pos = lds_find(cols, n, B.ci[p]);
-      vals[pos] = acc_step(vals[pos], a, B.val[p]);
+  const int n = active ? c_rp[row + 1] - c_s : 0;
+  int base;
+  int s0 = 0, s1 = 0;
+  if (seg) {
+    s0 = c_rp[first];
+    s1 = c_rp[last + 1];
+    for (int i = threadIdx.x; i < s1 - s0; i += kBlock) {
+      cols[i] = c_ci[s0 + i];
+      vals[i] = 0.0f;
+    }
+    __syncthreads();
+    base = c_s - s0;
+  } else {
+    base = g * cap;
+    for (int j = l; j < n; j += G) {
+      cols[base + j] = c_ci[c_s + j];
+      vals[base + j] = 0.0f;
     }
   }
-  for (int j = l; j < n; j += G) c_val[c_s + j] = vals[j];
+  if (active) {
+    const int* rc = cols + base;
+    float* rv = vals + base;
+    const int a_e = A.rp[row + 1];
+    for (int ap = A.rp[row]; ap < a_e; ++ap) {
+      const int k = A.ci[ap];
+      const float a = A.val[ap];
+      const int be = B.rp[k + 1];
+      for (int p = B.rp[k] + l; p < be; p += G) {
+        const int pos = lds_find(rc, n, B.ci[p]);
+        rv[pos] = acc_step(rv[pos], a, B.val[p]);
+      }
+    }
+  }
+  if (seg) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < s1 - s0; i += kBlock) c_val[s0 + i] = vals[i];
+  } else {
+    for (int j = l; j < n; j += G) c_val[c_s + j] = vals[base + j];
+  }
 }
 
 // First index in the ascending b_ci[s, e) whose value is >= key.

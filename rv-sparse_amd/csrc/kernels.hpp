@@ -106,6 +106,15 @@ hipError_t launch_classify_nnz(const DevCsr& A, RowStats st, const int* c_rp,
 hipError_t launch_partition(int rows, int nbins, PartitionScratch& ps,
                             int* rows_out, hipStream_t s);
 
+// Per-bin statistics of a partitioned phase: out[b * kBinStats + s] with
+// s = rows, sum nnz(A row), sum Op_i, sum size metric, max size metric,
+// sum span, max longest B row, max Op_i. `offsets` is the device copy of the
+// phase's bin offsets; c_rp is null for the count phase.
+constexpr int kBinStats = 8;
+hipError_t launch_bin_stats(const int* rows, const int* offsets, int nbins, int total_rows,
+                            const int* a_rp, RowStats st, const int* c_rp,
+                            unsigned long long* out, hipStream_t s);
+
 // --- symbolic -----------------------------------------------------------
 hipError_t launch_count_bin(const BinSpec& b, const int* rows, int nrows,
                             const DevCsr& A, const DevCsr& B, RowStats st,

@@ -100,6 +100,11 @@ PROTOTYPES = {
     "rvsp_spgemm_set_async": (C.c_int, [P, C.c_int]),
     "rvsp_spgemm_get_plan": (C.c_int, [P, C.POINTER(BinInfo), I32, C.POINTER(I32)]),
     "rvsp_spgemm_set_bin_choices": (C.c_int, [P, C.POINTER(BinChoice), I32]),
+    "rvsp_spgemm_autotune": (C.c_int, [P, PM, PM, PM, P, I32]),
+    "rvsp_tuning_table_save": (I32, [C.c_char_p]),
+    "rvsp_tuning_table_load": (I32, [C.c_char_p]),
+    "rvsp_tuning_table_clear": (None, []),
+    "rvsp_tuning_table_size": (I32, []),
     "rvsp_hip_malloc": (C.c_int, [C.POINTER(P), SZ]),
     "rvsp_hip_free": (C.c_int, [P]),
     "rvsp_hip_memcpy": (C.c_int, [P, P, SZ, C.c_int]),
