@@ -152,7 +152,19 @@ def main():
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--only", choices=["fixed", "tuned"], default=None,
                     help="time one plan only (profiling runs)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run device-to-device copies of this many MiB (a known "
+                         "byte count for scaling the HBM traffic counters), then exit")
     args = ap.parse_args()
+    if args.calibrate_copy_mib:
+        import rvsp
+        n = args.calibrate_copy_mib << 20
+        src, dst = rvsp.device_alloc(n), rvsp.device_alloc(n)
+        for _ in range(3):
+            rvsp._ffi.check(rvsp.lib().rvsp_hip_memcpy(C.c_void_p(dst), C.c_void_p(src), n, 3))
+        rvsp.synchronize()
+        print(json.dumps({"calibration_copy_bytes": n, "copies": 3}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X, and "
               "the SpGEMM path has no multi-GPU mode (DESIGN.md describes how it would "
