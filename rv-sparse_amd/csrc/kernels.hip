@@ -839,15 +839,44 @@ __global__ __launch_bounds__(kBlock) void k_numeric_search(
   if (active) {
     const int* rc = cols + base;
     float* rv = vals + base;
+    const int a_s = A.rp[row];
     const int a_e = A.rp[row + 1];
-    for (int ap = A.rp[row]; ap < a_e; ++ap) {
-      const int k = A.ci[ap];
-      const float a = A.val[ap];
-      const int be = B.rp[k + 1];
-      for (int p = B.rp[k] + l; p < be; p += G) {
-        const int pos = lds_find(rc, n, B.ci[p]);
-        rv[pos] = acc_step(rv[pos], a, B.val[p]);
+    // Software pipeline over A's row: while the products of entry ap are
+    // applied, the B-row bounds of ap+1 and the column of ap+2 are already
+    // in flight, so each entry costs one memory latency instead of three
+    // dependent ones. Only the loads run ahead; the additions stay in order.
+    int k1 = a_s + 1 < a_e ? A.ci[a_s + 1] : 0;
+    float a0 = a_s < a_e ? A.val[a_s] : 0.0f;
+    int bs0 = 0, be0 = 0;
+    if (a_s < a_e) {
+      const int k0 = A.ci[a_s];
+      bs0 = B.rp[k0];
+      be0 = B.rp[k0 + 1];
+    }
+    for (int ap = a_s; ap < a_e; ++ap) {
+      const bool has1 = ap + 1 < a_e;
+      const int bs1 = has1 ? B.rp[k1] : 0;
+      const int be1 = has1 ? B.rp[k1 + 1] : 0;
+      const float a1 = has1 ? A.val[ap + 1] : 0.0f;
+      const int k2 = ap + 2 < a_e ? A.ci[ap + 2] : 0;
+      // Two products per lane per step: their loads and searches overlap.
+      // They belong to one A entry, so their columns differ.
+      for (int p = bs0 + l; p < be0; p += 2 * G) {
+        const int q = p + G;
+        const bool hq = q < be0;
+        const int cp = B.ci[p];
+        const float vp = B.val[p];
+        const int cq = hq ? B.ci[q] : cp;
+        const float vq = hq ? B.val[q] : 0.0f;
+        const int pp = lds_find(rc, n, cp);
+        const int pq = lds_find(rc, n, cq);
+        rv[pp] = acc_step(rv[pp], a0, vp);
+        if (hq) rv[pq] = acc_step(rv[pq], a0, vq);
       }
+      bs0 = bs1;
+      be0 = be1;
+      a0 = a1;
+      k1 = k2;
     }
   }
   if (seg) {
