@@ -818,6 +818,41 @@ __global__ __launch_bounds__(kBlock) void k_numeric_search(
   const int row = active ? rows[r0 + g] : first;
   const int c_s = c_rp[row];
   const int n = active ? c_rp[row + 1] - c_s : 0;
+
+  // Software pipeline over A's row, primed before the C columns are staged
+  // so its first loads overlap the staging. At the top of step ap the lane
+  // holds entry ap's products (up to kU per lane, in registers), entry
+  // ap+1's scale and B-row bounds, and entry ap+2's column; during the step
+  // it loads entry ap+1's products, ap+2's bounds and ap+3's column. Only
+  // the loads run ahead: the additions into any C(i,j) stay in A order.
+  constexpr int kU = 2;
+  const int a_s = active ? A.rp[row] : 0;
+  const int a_e = active ? A.rp[row + 1] : 0;
+  int k1 = 0, k2 = 0, bs0 = 0, be0 = 0, bs1 = 0, be1 = 0;
+  float a0 = 0.0f, a1 = 0.0f;
+  if (a_s < a_e) {
+    const int k0 = A.ci[a_s];
+    a0 = A.val[a_s];
+    bs0 = B.rp[k0];
+    be0 = B.rp[k0 + 1];
+  }
+  if (a_s + 1 < a_e) {
+    k1 = A.ci[a_s + 1];
+    a1 = A.val[a_s + 1];
+    bs1 = B.rp[k1];
+    be1 = B.rp[k1 + 1];
+  }
+  if (a_s + 2 < a_e) k2 = A.ci[a_s + 2];
+  int c0[kU], c1[kU];
+  float v0[kU], v1[kU];
+#pragma unroll
+  for (int j = 0; j < kU; ++j) {
+    const int p = bs0 + l + j * G;
+    const bool h = p < be0;
+    c0[j] = h ? B.ci[p] : 0;
+    v0[j] = h ? B.val[p] : 0.0f;
+  }
+
   int base;
   int s0 = 0, s1 = 0;
   if (seg) {
@@ -839,44 +874,46 @@ __global__ __launch_bounds__(kBlock) void k_numeric_search(
   if (active) {
     const int* rc = cols + base;
     float* rv = vals + base;
-    const int a_s = A.rp[row];
-    const int a_e = A.rp[row + 1];
-    // Software pipeline over A's row: while the products of entry ap are
-    // applied, the B-row bounds of ap+1 and the column of ap+2 are already
-    // in flight, so each entry costs one memory latency instead of three
-    // dependent ones. Only the loads run ahead; the additions stay in order.
-    int k1 = a_s + 1 < a_e ? A.ci[a_s + 1] : 0;
-    float a0 = a_s < a_e ? A.val[a_s] : 0.0f;
-    int bs0 = 0, be0 = 0;
-    if (a_s < a_e) {
-      const int k0 = A.ci[a_s];
-      bs0 = B.rp[k0];
-      be0 = B.rp[k0 + 1];
-    }
     for (int ap = a_s; ap < a_e; ++ap) {
-      const bool has1 = ap + 1 < a_e;
-      const int bs1 = has1 ? B.rp[k1] : 0;
-      const int be1 = has1 ? B.rp[k1 + 1] : 0;
-      const float a1 = has1 ? A.val[ap + 1] : 0.0f;
-      const int k2 = ap + 2 < a_e ? A.ci[ap + 2] : 0;
-      // Two products per lane per step: their loads and searches overlap.
-      // They belong to one A entry, so their columns differ.
-      for (int p = bs0 + l; p < be0; p += 2 * G) {
-        const int q = p + G;
-        const bool hq = q < be0;
-        const int cp = B.ci[p];
-        const float vp = B.val[p];
-        const int cq = hq ? B.ci[q] : cp;
-        const float vq = hq ? B.val[q] : 0.0f;
-        const int pp = lds_find(rc, n, cp);
-        const int pq = lds_find(rc, n, cq);
-        rv[pp] = acc_step(rv[pp], a0, vp);
-        if (hq) rv[pq] = acc_step(rv[pq], a0, vq);
+      // Loads for the entries ahead.
+#pragma unroll
+      for (int j = 0; j < kU; ++j) {
+        const int p = bs1 + l + j * G;
+        const bool h = p < be1;
+        c1[j] = h ? B.ci[p] : 0;
+        v1[j] = h ? B.val[p] : 0.0f;
+      }
+      const bool has2 = ap + 2 < a_e;
+      const int bs2 = has2 ? B.rp[k2] : 0;
+      const int be2 = has2 ? B.rp[k2 + 1] : 0;
+      const float a2 = has2 ? A.val[ap + 2] : 0.0f;
+      const int k3 = ap + 3 < a_e ? A.ci[ap + 3] : 0;
+      // Entry ap: the products held in registers, then any remainder of a
+      // B row longer than kU * G. All belong to one A entry, so their
+      // columns differ and their relative order is free.
+#pragma unroll
+      for (int j = 0; j < kU; ++j) {
+        if (bs0 + l + j * G < be0) {
+          const int pos = lds_find(rc, n, c0[j]);
+          rv[pos] = acc_step(rv[pos], a0, v0[j]);
+        }
+      }
+      for (int p = bs0 + kU * G + l; p < be0; p += G) {
+        const int pos = lds_find(rc, n, B.ci[p]);
+        rv[pos] = acc_step(rv[pos], a0, B.val[p]);
+      }
+#pragma unroll
+      for (int j = 0; j < kU; ++j) {
+        c0[j] = c1[j];
+        v0[j] = v1[j];
       }
       bs0 = bs1;
       be0 = be1;
       a0 = a1;
-      k1 = k2;
+      bs1 = bs2;
+      be1 = be2;
+      a1 = a2;
+      k2 = k3;
     }
   }
   if (seg) {
