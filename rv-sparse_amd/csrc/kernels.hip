@@ -792,12 +792,19 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
 // accumulator; each product finds its column by binary search. A entries are
 // visited in order and the lanes of the group split the current B row only.
 //
-// When the workgroup's rows are consecutive (always for a bin holding a
-// banded or uniform matrix, whose rows stay in ascending order), its C rows
-// form one contiguous segment: the workgroup then loads the columns and
-// stores the values of the whole segment with coalesced accesses, and each
-// group works on its row's slice of the segment in LDS. Otherwise every
-// group stages its own row in a fixed slot of `cap` entries.
+// Segment mode. When the workgroup's rows are consecutive (always for a bin
+// holding a banded or uniform matrix, whose rows stay in ascending order),
+// its C rows and its A rows are each one contiguous range. The workgroup
+// then stages, with coalesced loads and without the G-fold redundancy of
+// per-group loads: the C and A row offsets, the C columns (and zeroed
+// accumulators), and for every A entry its scale and B-row bounds. The
+// per-entry loop is left with only the gathers of B's columns and values,
+// and the values are stored back coalesced. Otherwise every group stages
+// its own row in a fixed slot of `cap` entries and reads A directly.
+//
+// Loads run ahead of the arithmetic: entry ap+1's products (kU per lane,
+// in registers) are in flight while entry ap is applied. Only loads move;
+// the additions into any C(i,j) stay in A order.
 template <int G>
 __global__ __launch_bounds__(kBlock) void k_numeric_search(
     const int* __restrict__ rows, int nrows, int cap, DevCsr A, DevCsr B,
@@ -805,8 +812,16 @@ __global__ __launch_bounds__(kBlock) void k_numeric_search(
     float* __restrict__ c_val) {
   extern __shared__ int lds[];
   constexpr int kGroups = kBlock / G;
+  constexpr int kACap = kGroups * kSegAPerRow;
+  constexpr int kU = 2;
   int* cols = lds;
   float* vals = reinterpret_cast<float*>(lds + kGroups * cap);
+  int* seg_crp = lds + 2 * kGroups * cap;  // kGroups + 1 entries
+  int* seg_arp = seg_crp + kGroups + 1;     // kGroups + 1 entries
+  float* m_a = reinterpret_cast<float*>(seg_arp + kGroups + 1);  // kACap each
+  int* m_bs = reinterpret_cast<int*>(m_a + kACap);
+  int* m_be = m_bs + kACap;
+
   const int r0 = xcd_block(blockIdx.x, gridDim.x) * kGroups;
   const int nr = min(kGroups, nrows - r0);
   const int g = threadIdx.x / G;
@@ -815,34 +830,71 @@ __global__ __launch_bounds__(kBlock) void k_numeric_search(
   const int first = rows[r0];
   const int last = rows[r0 + nr - 1];
   const bool seg = (last - first == nr - 1);  // uniform over the block
-  const int row = active ? rows[r0 + g] : first;
-  const int c_s = c_rp[row];
-  const int n = active ? c_rp[row + 1] - c_s : 0;
 
-  // Software pipeline over A's row, primed before the C columns are staged
-  // so its first loads overlap the staging. At the top of step ap the lane
-  // holds entry ap's products (up to kU per lane, in registers), entry
-  // ap+1's scale and B-row bounds, and entry ap+2's column; during the step
-  // it loads entry ap+1's products, ap+2's bounds and ap+3's column. Only
-  // the loads run ahead: the additions into any C(i,j) stay in A order.
-  constexpr int kU = 2;
-  const int a_s = active ? A.rp[row] : 0;
-  const int a_e = active ? A.rp[row + 1] : 0;
-  int k1 = 0, k2 = 0, bs0 = 0, be0 = 0, bs1 = 0, be1 = 0;
+  int base, n, e_s, e_e;  // C slot base and length; A entries [e_s, e_e)
+  bool staged = false;    // A metadata in LDS, indexed from 0
+  int s0 = 0, s1 = 0;
+  if (seg) {
+    for (int i = threadIdx.x; i <= nr; i += kBlock) {
+      seg_crp[i] = c_rp[first + i];
+      seg_arp[i] = A.rp[first + i];
+    }
+    __syncthreads();
+    s0 = seg_crp[0];
+    s1 = seg_crp[nr];
+    const int as0 = seg_arp[0];
+    const int na = seg_arp[nr] - as0;
+    staged = na <= kACap;
+    for (int i = threadIdx.x; i < s1 - s0; i += kBlock) {
+      cols[i] = c_ci[s0 + i];
+      vals[i] = 0.0f;
+    }
+    if (staged) {
+      for (int i = threadIdx.x; i < na; i += kBlock) {
+        const int k = A.ci[as0 + i];
+        m_a[i] = A.val[as0 + i];
+        m_bs[i] = B.rp[k];
+        m_be[i] = B.rp[k + 1];
+      }
+    }
+    __syncthreads();
+    base = active ? seg_crp[g] - s0 : 0;
+    n = active ? seg_crp[g + 1] - seg_crp[g] : 0;
+    e_s = active ? seg_arp[g] - (staged ? as0 : 0) : 0;
+    e_e = active ? seg_arp[g + 1] - (staged ? as0 : 0) : 0;
+  } else {
+    const int row = active ? rows[r0 + g] : first;
+    const int c_s = c_rp[row];
+    base = g * cap;
+    n = active ? c_rp[row + 1] - c_s : 0;
+    e_s = active ? A.rp[row] : 0;
+    e_e = active ? A.rp[row + 1] : 0;
+    for (int j = l; j < n; j += G) {
+      cols[base + j] = c_ci[c_s + j];
+      vals[base + j] = 0.0f;
+    }
+  }
+
+  // Scale and B-row bounds of A entry e.
+  auto meta = [&](int e, float* a, int* bs, int* be) {
+    if (staged) {
+      *a = m_a[e];
+      *bs = m_bs[e];
+      *be = m_be[e];
+    } else {
+      const int k = A.ci[e];
+      *a = A.val[e];
+      *bs = B.rp[k];
+      *be = B.rp[k + 1];
+    }
+  };
+
+  const int* rc = cols + base;
+  float* rv = vals + base;
   float a0 = 0.0f, a1 = 0.0f;
-  if (a_s < a_e) {
-    const int k0 = A.ci[a_s];
-    a0 = A.val[a_s];
-    bs0 = B.rp[k0];
-    be0 = B.rp[k0 + 1];
-  }
-  if (a_s + 1 < a_e) {
-    k1 = A.ci[a_s + 1];
-    a1 = A.val[a_s + 1];
-    bs1 = B.rp[k1];
-    be1 = B.rp[k1 + 1];
-  }
-  if (a_s + 2 < a_e) k2 = A.ci[a_s + 2];
+  int bs0 = 0, be0 = 0, bs1 = 0, be1 = 0;
+  if (e_s < e_e) meta(e_s, &a0, &bs0, &be0);
+  if (e_s + 1 < e_e) meta(e_s + 1, &a1, &bs1, &be1);
   int c0[kU], c1[kU];
   float v0[kU], v1[kU];
 #pragma unroll
@@ -852,75 +904,50 @@ __global__ __launch_bounds__(kBlock) void k_numeric_search(
     c0[j] = h ? B.ci[p] : 0;
     v0[j] = h ? B.val[p] : 0.0f;
   }
+  for (int e = e_s; e < e_e; ++e) {
+#pragma unroll
+    for (int j = 0; j < kU; ++j) {
+      const int p = bs1 + l + j * G;
+      const bool h = p < be1;
+      c1[j] = h ? B.ci[p] : 0;
+      v1[j] = h ? B.val[p] : 0.0f;
+    }
+    float a2 = 0.0f;
+    int bs2 = 0, be2 = 0;
+    if (e + 2 < e_e) meta(e + 2, &a2, &bs2, &be2);
+    // Entry e: the products held in registers, then any remainder of a B row
+    // longer than kU * G. All belong to one A entry, so their columns
+    // differ and their relative order is free.
+#pragma unroll
+    for (int j = 0; j < kU; ++j) {
+      if (bs0 + l + j * G < be0) {
+        const int pos = lds_find(rc, n, c0[j]);
+        rv[pos] = acc_step(rv[pos], a0, v0[j]);
+      }
+    }
+    for (int p = bs0 + kU * G + l; p < be0; p += G) {
+      const int pos = lds_find(rc, n, B.ci[p]);
+      rv[pos] = acc_step(rv[pos], a0, B.val[p]);
+    }
+#pragma unroll
+    for (int j = 0; j < kU; ++j) {
+      c0[j] = c1[j];
+      v0[j] = v1[j];
+    }
+    a0 = a1;
+    bs0 = bs1;
+    be0 = be1;
+    a1 = a2;
+    bs1 = bs2;
+    be1 = be2;
+  }
 
-  int base;
-  int s0 = 0, s1 = 0;
-  if (seg) {
-    s0 = c_rp[first];
-    s1 = c_rp[last + 1];
-    for (int i = threadIdx.x; i < s1 - s0; i += kBlock) {
-      cols[i] = c_ci[s0 + i];
-      vals[i] = 0.0f;
-    }
-    __syncthreads();
-    base = c_s - s0;
-  } else {
-    base = g * cap;
-    for (int j = l; j < n; j += G) {
-      cols[base + j] = c_ci[c_s + j];
-      vals[base + j] = 0.0f;
-    }
-  }
-  if (active) {
-    const int* rc = cols + base;
-    float* rv = vals + base;
-    for (int ap = a_s; ap < a_e; ++ap) {
-      // Loads for the entries ahead.
-#pragma unroll
-      for (int j = 0; j < kU; ++j) {
-        const int p = bs1 + l + j * G;
-        const bool h = p < be1;
-        c1[j] = h ? B.ci[p] : 0;
-        v1[j] = h ? B.val[p] : 0.0f;
-      }
-      const bool has2 = ap + 2 < a_e;
-      const int bs2 = has2 ? B.rp[k2] : 0;
-      const int be2 = has2 ? B.rp[k2 + 1] : 0;
-      const float a2 = has2 ? A.val[ap + 2] : 0.0f;
-      const int k3 = ap + 3 < a_e ? A.ci[ap + 3] : 0;
-      // Entry ap: the products held in registers, then any remainder of a
-      // B row longer than kU * G. All belong to one A entry, so their
-      // columns differ and their relative order is free.
-#pragma unroll
-      for (int j = 0; j < kU; ++j) {
-        if (bs0 + l + j * G < be0) {
-          const int pos = lds_find(rc, n, c0[j]);
-          rv[pos] = acc_step(rv[pos], a0, v0[j]);
-        }
-      }
-      for (int p = bs0 + kU * G + l; p < be0; p += G) {
-        const int pos = lds_find(rc, n, B.ci[p]);
-        rv[pos] = acc_step(rv[pos], a0, B.val[p]);
-      }
-#pragma unroll
-      for (int j = 0; j < kU; ++j) {
-        c0[j] = c1[j];
-        v0[j] = v1[j];
-      }
-      bs0 = bs1;
-      be0 = be1;
-      a0 = a1;
-      bs1 = bs2;
-      be1 = be2;
-      a1 = a2;
-      k2 = k3;
-    }
-  }
   if (seg) {
     __syncthreads();
     for (int i = threadIdx.x; i < s1 - s0; i += kBlock) c_val[s0 + i] = vals[i];
-  } else {
-    for (int j = l; j < n; j += G) c_val[c_s + j] = vals[base + j];
+  } else if (active) {
+    const int c_s = c_rp[rows[r0 + g]];
+    for (int j = l; j < n; j += G) c_val[c_s + j] = rv[j];
   }
 }
 

@@ -139,4 +139,8 @@ hipError_t init_kernel_attributes(int lds_bytes);
 // Largest dynamic LDS a workgroup may request on gfx950.
 constexpr int kMaxLdsBytes = 160 * 1024;
 
+// A entries per row the SEARCH kernel's segment mode stages in LDS (a
+// workgroup whose rows hold more A entries reads them from global memory).
+constexpr int kSegAPerRow = 6;
+
 }  // namespace rvsp

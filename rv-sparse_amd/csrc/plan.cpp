@@ -63,7 +63,10 @@ int lds_bytes_for(int phase, int kind, int group, int cap) {
       break;
     case KIND_SEARCH:
       if (phase != PHASE_NUMERIC || !sub_wave || cap < 1) break;
-      bytes = (long long)(kBlock / group) * cap * 8;
+      // C columns + accumulators, the segment's C and A row offsets, and
+      // the staged A-entry metadata (scale, B-row start, B-row end).
+      bytes = (long long)(kBlock / group) * cap * 8 + 2LL * (kBlock / group + 1) * 4 +
+              (long long)(kBlock / group) * kSegAPerRow * 12;
       break;
     case KIND_BITMAP:
       if (phase == PHASE_NUMERIC || cap < 32 || (cap & 31)) break;
