@@ -69,7 +69,12 @@ typedef enum
     RVSP_KIND_SEARCH = 3,  /* LDS sorted columns + binary search */
     RVSP_KIND_BITMAP = 4,  /* LDS bitmap over the row's column span */
     RVSP_KIND_BLOCK = 5,   /* workgroup per row, LDS, waves own column ranges */
-    RVSP_KIND_GLOBAL = 6   /* workgroup per row, dense global scratch */
+    RVSP_KIND_GLOBAL = 6,  /* workgroup per row, dense global scratch */
+    RVSP_KIND_MAP = 7      /* numeric via a cached position per product: the
+                              descriptor records, on the first compute, where
+                              in C's row every product lands (1 or 2 bytes
+                              per product) and later numeric-only computes
+                              read that instead of searching */
 } rvsp_kind_t;
 
 /* One bin of the analysed plan. */

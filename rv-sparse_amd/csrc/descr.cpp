@@ -7,6 +7,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <climits>
 #include <mutex>
 #include <vector>
@@ -129,6 +130,9 @@ void drop_structure(rvsp_spgemm_descr* d) {
   d->filled_into = NULL;
   d->rowptr_into = NULL;
   d->dev_cols_valid = false;
+  d->map_valid = false;
+  d->map_bins = 0;
+  d->map_width = 0;
   d->estimated = 0;
   d->c_nnz = 0;
 }
@@ -343,6 +347,7 @@ rvsp_status_t estimate(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
   RVSP_TRY(fetch_offsets(d, fp, 2, 1, s));
   RVSP_TRY(select_phase(d, d->fill, PHASE_FILL, A, (const int*)d->c_rp.p, s));
   RVSP_TRY(select_phase(d, d->numeric, PHASE_NUMERIC, A, (const int*)d->c_rp.p, s));
+  RVSP_TRY(plan_map(d, s));
   RVSP_TRY(ensure_slots(d, d->fill, B.cols, false, s));
   RVSP_TRY(ensure_slots(d, d->numeric, B.cols, true, s));
   return RVSP_SUCCESS;
@@ -371,10 +376,78 @@ rvsp_status_t run_numeric(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B
   for (int b = 1; b < d->numeric.table.rule.nbins; ++b) {
     const int n = bin_rows(d->numeric, b);
     if (n == 0) continue;
-    RVSP_HIP_TRY(launch_numeric_bin(
-        d->numeric.table.spec[b], (const int*)d->numeric.rows.p + d->numeric.offsets[b],
-        n, A, B, st, (const int*)d->c_rp.p, c_ci, c_val, d->gs_dense, s));
+    const BinSpec& spec = d->numeric.table.spec[b];
+    const int* rows = (const int*)d->numeric.rows.p + d->numeric.offsets[b];
+    if (spec.kind == KIND_MAP) {
+      RVSP_TRY(ensure_map_built(d, A, B, c_ci, s));
+      MapView mv;
+      mv.map = d->map.p;
+      mv.poff = (const long long*)d->poff.p;
+      mv.width = d->map_width;
+      RVSP_HIP_TRY(launch_numeric_map_bin(spec, mv, rows, n, A, B, (const int*)d->c_rp.p,
+                                          c_val, s));
+    } else {
+      RVSP_HIP_TRY(launch_numeric_bin(spec, rows, n, A, B, st, (const int*)d->c_rp.p, c_ci,
+                                      c_val, d->gs_dense, s));
+    }
   }
+  return RVSP_SUCCESS;
+}
+
+rvsp_status_t plan_map(rvsp_spgemm_descr* d, hipStream_t s) {
+  PhasePlan& p = d->numeric;
+  unsigned mask = 0;
+  int max_bound = 0;
+  for (int b = 2; b < p.table.rule.nbins; ++b) {
+    if (p.table.spec[b].kind != KIND_MAP || bin_rows(p, b) == 0) continue;
+    mask |= 1u << b;
+    max_bound = max_bound > p.table.rule.bound[b] ? max_bound : p.table.rule.bound[b];
+  }
+  d->map_valid = false;
+  d->map_bins = 0;
+  d->map_width = 0;
+  if (mask == 0) return RVSP_SUCCESS;
+  const int M = p.offsets.empty() ? 0 : p.offsets.back();  // rows partitioned
+  const int width = max_bound <= 256 ? 1 : 2;  // positions are < the row's nnz
+  RVSP_TRY(ensure(d->poff, ((size_t)M + 1) * 8));
+  RVSP_TRY(ensure(d->map_weights, (size_t)M * 8));
+  long long total = 0;
+  RVSP_HIP_TRY(launch_map_offsets(M, (const unsigned char*)d->bin_of.p, mask, row_stats(d),
+                                  (long long*)d->map_weights.p, (long long*)d->poff.p,
+                                  (long long*)d->scan_tiles.p, (long long*)d->total.p, s));
+  RVSP_HIP_TRY(hipMemcpyAsync(&total, d->total.p, 8, hipMemcpyDeviceToHost, s));
+  RVSP_HIP_TRY(hipStreamSynchronize(s));
+  const size_t bytes = (size_t)total * width;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  const size_t budget = std::min<size_t>((size_t)8 << 30, free_b / 4);
+  if (bytes > budget || ensure(d->map, bytes) != RVSP_SUCCESS) {
+    // Over budget: these bins keep searching C's columns instead.
+    for (int b = 2; b < p.table.rule.nbins; ++b)
+      if ((mask >> b) & 1u) {
+        BinChoice c;
+        c.kind = KIND_SEARCH;
+        apply_choice(p.table, PHASE_NUMERIC, b, c, p.table.rule.bound[b]);
+      }
+    return RVSP_SUCCESS;
+  }
+  d->map_bins = mask;
+  d->map_width = width;
+  return RVSP_SUCCESS;
+}
+
+rvsp_status_t ensure_map_built(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
+                               const int* c_ci, hipStream_t s) {
+  if (d->map_valid) return RVSP_SUCCESS;
+  if (d->map_width == 0) return RVSP_ERROR_DEVICE;  // a MAP bin without a planned map
+  PhasePlan& p = d->numeric;
+  for (int b = 2; b < p.table.rule.nbins; ++b) {
+    if (!((d->map_bins >> b) & 1u)) continue;
+    RVSP_HIP_TRY(launch_build_map(d->map_width, (const int*)p.rows.p + p.offsets[b],
+                                  bin_rows(p, b), A, B, (const int*)d->c_rp.p, c_ci,
+                                  (const long long*)d->poff.p, d->map.p, s));
+  }
+  d->map_valid = true;
   return RVSP_SUCCESS;
 }
 
@@ -404,7 +477,8 @@ void rvsp_spgemm_descr_destroy(rvsp_spgemm_descr_t descr) {
                     &d->cnt, &d->c_rp, &d->scan_tiles, &d->total, &d->bin_of,
                     &d->tile_hist, &d->tile_base, &d->bin_offsets, &d->count.rows,
                     &d->fill.rows, &d->numeric.rows, &d->c_ci_dev, &d->c_val_dev,
-                    &d->gbits, &d->gdense, &d->learn_feat, &d->bin_stats};
+                    &d->gbits, &d->gdense, &d->learn_feat, &d->bin_stats,
+                    &d->map, &d->poff, &d->map_weights};
   for (DevBuf* b : bufs) release(*b);
   destroy_graph(d);
   if (d->own_stream) (void)hipStreamDestroy(d->own_stream);
@@ -427,6 +501,7 @@ void rvsp_spgemm_invalidate_structure(rvsp_spgemm_descr_t descr) {
   if (descr != NULL) {
     descr->filled_into = NULL;
     descr->dev_cols_valid = false;
+    descr->map_valid = false;
   }
 }
 

@@ -75,6 +75,13 @@ struct rvsp_spgemm_descr {
   // Per-bin feature scratch (tuned and learned selectors).
   rvsp::DevBuf learn_feat, bin_stats;
 
+  // Position map of the MAP numeric bins (structure cache for repeated
+  // numeric-only computes): offsets, entries, coverage and validity.
+  rvsp::DevBuf map, poff, map_weights;
+  int map_width = 0;        // bytes per entry; 0 = no map planned
+  unsigned map_bins = 0;    // numeric bins whose rows have map entries
+  bool map_valid = false;   // entries built for the current structure
+
   // Captured numeric-only graph.
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -89,6 +96,13 @@ rvsp_status_t run_fill(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
                        int* c_ci, hipStream_t s);
 rvsp_status_t run_numeric(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
                           const int* c_ci, float* c_val, hipStream_t s);
+
+// Plans the position map for the numeric bins of kind MAP (offsets and
+// storage); bins that do not fit the memory budget fall back to SEARCH.
+rvsp_status_t plan_map(rvsp_spgemm_descr* d, hipStream_t s);
+// Builds the map entries if they are not valid for the current structure.
+rvsp_status_t ensure_map_built(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
+                               const int* c_ci, hipStream_t s);
 
 void destroy_graph(rvsp_spgemm_descr* d);
 

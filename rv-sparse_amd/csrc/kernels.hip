@@ -951,6 +951,271 @@ __global__ __launch_bounds__(kBlock) void k_numeric_search(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Position map: a structure cache for repeated numeric-only multiplies.
+//
+// Once C's sorted columns exist, the position inside C's row that each
+// product (row i, A entry ap, B entry bp) lands on is fixed for as long as
+// the sparsity pattern is. map[poff[i] + t] holds it for the t-th product
+// of row i in (A entry, B entry) order, as uint8 when every mapped row has
+// at most 256 columns, else uint16. A numeric pass with the map reads one
+// byte or two per product instead of B's column and C's columns, and does
+// no search. poff is an exclusive scan of Op_i over the mapped rows.
+
+// w[i] = Op_i for rows whose numeric bin is in `bin_mask`, else 0.
+__global__ __launch_bounds__(kBlock) void k_map_weights(int M, const unsigned char* __restrict__ bin_of,
+                                                        unsigned bin_mask, RowStats st,
+                                                        long long* __restrict__ w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < M) w[i] = ((bin_mask >> bin_of[i]) & 1u) ? st.ops[i] : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan64_reduce(const long long* __restrict__ in, int n,
+                                                          long long* __restrict__ tile_sums) {
+  __shared__ long long tmp[kBlock / kWave];
+  const int base = blockIdx.x * kScanTile;
+  long long s = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const int i = base + j * kBlock + threadIdx.x;
+    if (i < n) s += in[i];
+  }
+  s = block_sum(s, tmp);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan64_apply(const long long* __restrict__ in, int n,
+                                                         const long long* __restrict__ tile_off,
+                                                         long long* __restrict__ out) {
+  __shared__ long long tmp[kBlock / kWave];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  long long v[kScanItems];
+  long long s = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    v[j] = (base + j < n) ? in[base + j] : 0;
+    s += v[j];
+  }
+  long long tot;
+  long long run = block_excl_scan(s, tmp, &tot) + tile_off[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    if (base + j < n) {
+      out[base + j] = run;
+      run += v[j];
+      if (base + j == n - 1) out[n] = run;
+    }
+  }
+}
+
+hipError_t launch_map_offsets(int M, const unsigned char* bin_of, unsigned bin_mask, RowStats st,
+                              long long* weights, long long* poff, long long* tile_sums,
+                              long long* total, hipStream_t s) {
+  if (M <= 0) return hipSuccess;
+  const int ntiles = (M + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_map_weights, dim3((M + kBlock - 1) / kBlock), dim3(kBlock), 0, s, M,
+                     bin_of, bin_mask, st, weights);
+  hipLaunchKernelGGL(k_scan64_reduce, dim3(ntiles), dim3(kBlock), 0, s, weights, M, tile_sums);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, s, tile_sums, ntiles, total);
+  hipLaunchKernelGGL(k_scan64_apply, dim3(ntiles), dim3(kBlock), 0, s, weights, M, tile_sums, poff);
+  return hipGetLastError();
+}
+
+// Builds the map entries of the listed rows: 16 lanes per row, positions by
+// binary search in C's sorted row (global memory; runs once per structure).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_build_map(const int* __restrict__ rows, int nrows,
+                                                      DevCsr A, DevCsr B,
+                                                      const int* __restrict__ c_rp,
+                                                      const int* __restrict__ c_ci,
+                                                      const long long* __restrict__ poff,
+                                                      T* __restrict__ map) {
+  constexpr int G = 16;
+  const int r = (blockIdx.x * kBlock + threadIdx.x) / G;
+  const int l = threadIdx.x & (G - 1);
+  if (r >= nrows) return;
+  const int row = rows[r];
+  const int c_s = c_rp[row];
+  const int n = c_rp[row + 1] - c_s;
+  const int* cc = c_ci + c_s;
+  long long mo = poff[row];
+  for (int ap = A.rp[row]; ap < A.rp[row + 1]; ++ap) {
+    const int k = A.ci[ap];
+    const int bs = B.rp[k], be = B.rp[k + 1];
+    for (int p = bs + l; p < be; p += G) map[mo + (p - bs)] = (T)lds_find(cc, n, B.ci[p]);
+    mo += be - bs;
+  }
+}
+
+hipError_t launch_build_map(int width, const int* rows, int nrows, const DevCsr& A,
+                            const DevCsr& B, const int* c_rp, const int* c_ci,
+                            const long long* poff, void* map, hipStream_t s) {
+  if (nrows <= 0) return hipSuccess;
+  const int grid = (int)(((long long)nrows * 16 + kBlock - 1) / kBlock);
+  if (width == 1)
+    hipLaunchKernelGGL(k_build_map<unsigned char>, dim3(grid), dim3(kBlock), 0, s, rows, nrows,
+                       A, B, c_rp, c_ci, poff, (unsigned char*)map);
+  else
+    hipLaunchKernelGGL(k_build_map<unsigned short>, dim3(grid), dim3(kBlock), 0, s, rows, nrows,
+                       A, B, c_rp, c_ci, poff, (unsigned short*)map);
+  return hipGetLastError();
+}
+
+// Numeric pass driven by the position map. Same row/group structure,
+// segment mode and load pipeline as k_numeric_search, but LDS holds only the
+// accumulators (and, in segment mode, the segment's slice of the map), and
+// each product is one map entry, one B value and one LDS add.
+template <int G, typename T>
+__global__ __launch_bounds__(kBlock) void k_numeric_map(
+    const int* __restrict__ rows, int nrows, int cap, DevCsr A, DevCsr B,
+    const int* __restrict__ c_rp, const long long* __restrict__ poff,
+    const T* __restrict__ map, float* __restrict__ c_val) {
+  extern __shared__ int lds[];
+  constexpr int kGroups = kBlock / G;
+  constexpr int kMapCap = kGroups * kSegMapPerRow;  // map entries staged per segment
+  constexpr int kU = 2;
+  float* vals = reinterpret_cast<float*>(lds);
+  T* mstage = reinterpret_cast<T*>(lds + kGroups * cap);
+
+  const int r0 = xcd_block(blockIdx.x, gridDim.x) * kGroups;
+  const int nr = min(kGroups, nrows - r0);
+  const int g = threadIdx.x / G;
+  const int l = threadIdx.x & (G - 1);
+  const bool active = g < nr;
+  const int first = rows[r0];
+  const int last = rows[r0 + nr - 1];
+  const bool seg = (last - first == nr - 1);
+  const int row = active ? rows[r0 + g] : first;
+  const int c_s = c_rp[row];
+  const int n = active ? c_rp[row + 1] - c_s : 0;
+  const long long mo = poff[row];
+  const int a_s = active ? A.rp[row] : 0;
+  const int a_e = active ? A.rp[row + 1] : 0;
+
+  // Load pipeline, primed before staging (see k_numeric_search).
+  int k1 = 0, k2 = 0, bs0 = 0, be0 = 0, bs1 = 0, be1 = 0;
+  float a0 = 0.0f, a1 = 0.0f;
+  if (a_s < a_e) {
+    const int k0 = A.ci[a_s];
+    a0 = A.val[a_s];
+    bs0 = B.rp[k0];
+    be0 = B.rp[k0 + 1];
+  }
+  if (a_s + 1 < a_e) {
+    k1 = A.ci[a_s + 1];
+    a1 = A.val[a_s + 1];
+    bs1 = B.rp[k1];
+    be1 = B.rp[k1 + 1];
+  }
+  if (a_s + 2 < a_e) k2 = A.ci[a_s + 2];
+  float v0[kU], v1[kU];
+#pragma unroll
+  for (int j = 0; j < kU; ++j) {
+    const int p = bs0 + l + j * G;
+    v0[j] = p < be0 ? B.val[p] : 0.0f;
+  }
+
+  int base;
+  int s0 = 0, s1 = 0;
+  const T* rm = map + mo;  // this row's map entries
+  if (seg) {
+    s0 = c_rp[first];
+    s1 = c_rp[last + 1];
+    const long long m0 = poff[first];
+    const long long m1 = poff[last + 1];
+    const bool staged = m1 - m0 <= kMapCap;  // uniform
+    for (int i = threadIdx.x; i < s1 - s0; i += kBlock) vals[i] = 0.0f;
+    if (staged)
+      for (int i = threadIdx.x; i < (int)(m1 - m0); i += kBlock) mstage[i] = map[m0 + i];
+    __syncthreads();
+    base = c_s - s0;
+    if (staged) rm = mstage + (mo - m0);
+  } else {
+    base = g * cap;
+    for (int j = l; j < n; j += G) vals[base + j] = 0.0f;
+  }
+
+  if (active) {
+    float* rv = vals + base;
+    int t0 = 0;  // index of entry ap's first product within the row
+    for (int ap = a_s; ap < a_e; ++ap) {
+#pragma unroll
+      for (int j = 0; j < kU; ++j) {
+        const int p = bs1 + l + j * G;
+        v1[j] = p < be1 ? B.val[p] : 0.0f;
+      }
+      const bool has2 = ap + 2 < a_e;
+      const int bs2 = has2 ? B.rp[k2] : 0;
+      const int be2 = has2 ? B.rp[k2 + 1] : 0;
+      const float a2 = has2 ? A.val[ap + 2] : 0.0f;
+      const int k3 = ap + 3 < a_e ? A.ci[ap + 3] : 0;
+      const int len = be0 - bs0;
+#pragma unroll
+      for (int j = 0; j < kU; ++j) {
+        const int t = l + j * G;
+        if (t < len) {
+          const int pos = rm[t0 + t];
+          rv[pos] = acc_step(rv[pos], a0, v0[j]);
+        }
+      }
+      for (int t = kU * G + l; t < len; t += G) {
+        const int pos = rm[t0 + t];
+        rv[pos] = acc_step(rv[pos], a0, B.val[bs0 + t]);
+      }
+      t0 += len;
+#pragma unroll
+      for (int j = 0; j < kU; ++j) v0[j] = v1[j];
+      a0 = a1;
+      bs0 = bs1;
+      be0 = be1;
+      a1 = a2;
+      bs1 = bs2;
+      be1 = be2;
+      k2 = k3;
+    }
+  }
+
+  if (seg) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < s1 - s0; i += kBlock) c_val[s0 + i] = vals[i];
+  } else {
+    for (int j = l; j < n; j += G) c_val[c_s + j] = vals[base + j];
+  }
+}
+
+template <int G>
+static hipError_t launch_numeric_map_g(const BinSpec& b, int width, const int* rows, int nrows,
+                                       const DevCsr& A, const DevCsr& B, const int* c_rp,
+                                       const long long* poff, const void* map, float* c_val,
+                                       hipStream_t s) {
+  const int groups = kBlock / G;
+  const int grid = (nrows + groups - 1) / groups;
+  if (width == 1)
+    hipLaunchKernelGGL((k_numeric_map<G, unsigned char>), dim3(grid), dim3(kBlock), b.lds_bytes,
+                       s, rows, nrows, b.cap, A, B, c_rp, poff, (const unsigned char*)map, c_val);
+  else
+    hipLaunchKernelGGL((k_numeric_map<G, unsigned short>), dim3(grid), dim3(kBlock), b.lds_bytes,
+                       s, rows, nrows, b.cap, A, B, c_rp, poff, (const unsigned short*)map, c_val);
+  return hipGetLastError();
+}
+
+hipError_t launch_numeric_map_bin(const BinSpec& b, const MapView& mv, const int* rows, int nrows,
+                                  const DevCsr& A, const DevCsr& B, const int* c_rp,
+                                  float* c_val, hipStream_t s) {
+  if (nrows <= 0) return hipSuccess;
+  if (!mv.map || !mv.poff || (mv.width != 1 && mv.width != 2)) return hipErrorInvalidValue;
+  switch (b.group) {
+    case 1: return launch_numeric_map_g<1>(b, mv.width, rows, nrows, A, B, c_rp, mv.poff, mv.map, c_val, s);
+    case 2: return launch_numeric_map_g<2>(b, mv.width, rows, nrows, A, B, c_rp, mv.poff, mv.map, c_val, s);
+    case 4: return launch_numeric_map_g<4>(b, mv.width, rows, nrows, A, B, c_rp, mv.poff, mv.map, c_val, s);
+    case 8: return launch_numeric_map_g<8>(b, mv.width, rows, nrows, A, B, c_rp, mv.poff, mv.map, c_val, s);
+    case 16: return launch_numeric_map_g<16>(b, mv.width, rows, nrows, A, B, c_rp, mv.poff, mv.map, c_val, s);
+    case 32: return launch_numeric_map_g<32>(b, mv.width, rows, nrows, A, B, c_rp, mv.poff, mv.map, c_val, s);
+    case 64: return launch_numeric_map_g<64>(b, mv.width, rows, nrows, A, B, c_rp, mv.poff, mv.map, c_val, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 // First index in the ascending b_ci[s, e) whose value is >= key.
 __device__ __forceinline__ int gm_lower_bound(const int* __restrict__ b_ci, int s,
                                               int e, int key) {

@@ -44,7 +44,7 @@ enum Kind : int {
   KIND_BITMAP = 4,   // one workgroup per row, LDS bitmap over the row's span (count, fill)
   KIND_BLOCK = 5,    // one workgroup per row, LDS sorted columns (numeric)
   KIND_GLOBAL = 6,   // one workgroup per row, dense scratch in global memory
-  KIND_DENSE = 7     // LDS dense accumulator over the row's column span (numeric)
+  KIND_MAP = 7       // position map + LDS accumulators, G lanes per row (numeric)
 };
 
 enum Phase : int { PHASE_COUNT = 0, PHASE_FILL = 1, PHASE_NUMERIC = 2 };
@@ -133,6 +133,24 @@ hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
                               const int* c_rp, const int* c_ci, float* c_val,
                               const GlobalSlots& gs, hipStream_t s);
 
+// --- position map (numeric structure cache) -------------------------------
+struct MapView {
+  const void* map = nullptr;        // uint8 or uint16 entries
+  const long long* poff = nullptr;  // rows + 1 entries
+  int width = 0;                    // bytes per entry (1 or 2); 0 = no map
+};
+// poff = exclusive scan over rows of (bin_of[i] in bin_mask ? Op_i : 0);
+// *total = poff[rows]. `weights` holds rows int64 values of scratch.
+hipError_t launch_map_offsets(int rows, const unsigned char* bin_of, unsigned bin_mask,
+                              RowStats st, long long* weights, long long* poff,
+                              long long* tile_sums, long long* total, hipStream_t s);
+hipError_t launch_build_map(int width, const int* rows, int nrows, const DevCsr& A,
+                            const DevCsr& B, const int* c_rp, const int* c_ci,
+                            const long long* poff, void* map, hipStream_t s);
+hipError_t launch_numeric_map_bin(const BinSpec& b, const MapView& mv, const int* rows,
+                                  int nrows, const DevCsr& A, const DevCsr& B,
+                                  const int* c_rp, float* c_val, hipStream_t s);
+
 // One-time kernel attribute setup: allows `lds_bytes` of dynamic LDS.
 hipError_t init_kernel_attributes(int lds_bytes);
 
@@ -142,5 +160,8 @@ constexpr int kMaxLdsBytes = 160 * 1024;
 // A entries per row the SEARCH kernel's segment mode stages in LDS (a
 // workgroup whose rows hold more A entries reads them from global memory).
 constexpr int kSegAPerRow = 6;
+
+// Map entries per row the MAP kernel's segment mode stages in LDS.
+constexpr int kSegMapPerRow = 32;
 
 }  // namespace rvsp

@@ -38,7 +38,7 @@ const char* kind_name(int kind) {
     case KIND_BITMAP: return "lds_bitmap";
     case KIND_BLOCK: return "lds_block";
     case KIND_GLOBAL: return "global_dense";
-    case KIND_DENSE: return "lds_dense";
+    case KIND_MAP: return "lds_map";
     default: return "?";
   }
 }
@@ -67,6 +67,11 @@ int lds_bytes_for(int phase, int kind, int group, int cap) {
       // the staged A-entry metadata (scale, B-row start, B-row end).
       bytes = (long long)(kBlock / group) * cap * 8 + 2LL * (kBlock / group + 1) * 4 +
               (long long)(kBlock / group) * kSegAPerRow * 12;
+      break;
+    case KIND_MAP:
+      if (phase != PHASE_NUMERIC || !sub_wave || cap < 1 || cap > 65536) break;
+      // Accumulators plus the staged slice of the map (up to 2-byte entries).
+      bytes = (long long)(kBlock / group) * cap * 4 + ((kBlock / group) * kSegMapPerRow * 2 + 3) / 4 * 4;
       break;
     case KIND_BITMAP:
       if (phase == PHASE_NUMERIC || cap < 32 || (cap & 31)) break;
@@ -151,6 +156,7 @@ bool apply_choice(PhaseTable& t, int phase, int b, const BinChoice& c,
         ok = phase != PHASE_NUMERIC && max_metric <= cand.cap / 2;
         break;
       case KIND_SEARCH:
+      case KIND_MAP:
         ok = phase == PHASE_NUMERIC && max_metric <= cand.cap;
         break;
       case KIND_BLOCK:

@@ -151,22 +151,34 @@ hipError_t launch_bin(rvsp_spgemm_descr* d, int phase, const BinSpec& spec, cons
       return launch_fill_bin(spec, rows, n, A, B, st, (const int*)d->c_rp.p, c_ci,
                              d->gs_bits, s);
     default:
+      if (spec.kind == KIND_MAP) {
+        MapView mv;
+        mv.map = d->map.p;
+        mv.poff = (const long long*)d->poff.p;
+        mv.width = d->map_width;
+        return launch_numeric_map_bin(spec, mv, rows, n, A, B, (const int*)d->c_rp.p, c_val, s);
+      }
       return launch_numeric_bin(spec, rows, n, A, B, st, (const int*)d->c_rp.p, c_ci, c_val,
                                 d->gs_dense, s);
   }
 }
 
-std::vector<BinChoice> candidates(const PhaseTable& t, int phase, int b) {
+std::vector<BinChoice> candidates(const PhaseTable& t, int phase, int b, bool map_ok) {
   std::vector<BinChoice> out;
   const BinSpec& cur = t.spec[b];
   const int groups[] = {1, 2, 4, 8, 16, 32, 64};
-  if (cur.kind == KIND_DIRECT || cur.kind == KIND_HASH || cur.kind == KIND_SEARCH) {
+  if (cur.kind == KIND_DIRECT || cur.kind == KIND_HASH || cur.kind == KIND_SEARCH ||
+      cur.kind == KIND_MAP) {
     for (int g : groups) {
       BinChoice c;
-      c.kind = cur.kind;
+      c.kind = cur.kind == KIND_MAP ? KIND_SEARCH : cur.kind;
       c.group = g;
       c.cap = cur.cap;
       out.push_back(c);
+      if (map_ok && (cur.kind == KIND_SEARCH || cur.kind == KIND_MAP)) {
+        c.kind = KIND_MAP;
+        out.push_back(c);
+      }
     }
   }
   if (phase == PHASE_NUMERIC && (cur.kind == KIND_SEARCH || cur.kind == KIND_BLOCK)) {
@@ -201,6 +213,25 @@ rvsp_status_t autotune_plan(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr&
   RVSP_HIP_TRY(hipEventCreate(&e1));
   PhasePlan* plans[] = {&d->count, &d->fill, &d->numeric};
   rvsp_status_t status = RVSP_SUCCESS;
+  // Give every searching numeric bin a position map when the memory budget
+  // allows, so searching and the map can be timed against each other.
+  {
+    PhaseTable& nt = d->numeric.table;
+    bool any = false;
+    for (int b = 2; b < nt.rule.nbins; ++b) {
+      const int n = d->numeric.offsets[b + 1] - d->numeric.offsets[b];
+      if (n == 0 || nt.spec[b].kind != KIND_SEARCH) continue;
+      BinChoice c;
+      c.kind = KIND_MAP;
+      PhaseTable trial = nt;  // a refused choice would turn the bin GLOBAL
+      if (apply_choice(trial, PHASE_NUMERIC, b, c, nt.rule.bound[b])) {
+        nt.spec[b] = trial.spec[b];
+        any = true;
+      }
+    }
+    if (any) status = plan_map(d, s);
+    if (status == RVSP_SUCCESS && d->map_width) status = ensure_map_built(d, A, B, c_ci, s);
+  }
   for (int ph = 0; ph < 3 && status == RVSP_SUCCESS; ++ph) {
     PhasePlan& p = *plans[ph];
     std::vector<BinFeatures> feats;
@@ -213,7 +244,8 @@ rvsp_status_t autotune_plan(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr&
       float best_us = 0.f;
       BinSpec best = p.table.spec[b];
       bool have = false;
-      for (const BinChoice& c : candidates(p.table, ph, b)) {
+      const bool map_ok = ph == PHASE_NUMERIC && ((d->map_bins >> b) & 1u) && d->map_valid;
+      for (const BinChoice& c : candidates(p.table, ph, b, map_ok)) {
         PhaseTable trial = p.table;
         if (!apply_choice(trial, ph, b, c, (int)feats[b].max_metric)) continue;
         const BinSpec& spec = trial.spec[b];

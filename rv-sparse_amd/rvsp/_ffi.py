@@ -33,10 +33,10 @@ MEMORY_HOST, MEMORY_DEVICE = 0, 1
 # rvsp_phase_t / rvsp_kind_t
 PHASE_COUNT, PHASE_FILL, PHASE_NUMERIC = 0, 1, 2
 (KIND_NONE, KIND_DIRECT, KIND_HASH, KIND_SEARCH, KIND_BITMAP, KIND_BLOCK,
- KIND_GLOBAL) = range(7)
+ KIND_GLOBAL, KIND_MAP) = range(8)
 KIND_NAMES = {KIND_NONE: "none", KIND_DIRECT: "direct", KIND_HASH: "lds_hash",
               KIND_SEARCH: "lds_search", KIND_BITMAP: "lds_bitmap",
-              KIND_BLOCK: "lds_block", KIND_GLOBAL: "global_dense"}
+              KIND_BLOCK: "lds_block", KIND_GLOBAL: "global_dense", KIND_MAP: "lds_map"}
 # rvsp_csr_status_t
 CSR_OK, CSR_BAD_ROW_PTR, CSR_COL_OUT_OF_RANGE, CSR_UNSORTED, CSR_DUPLICATE = range(5)
 

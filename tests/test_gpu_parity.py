@@ -101,7 +101,41 @@ PLAN_VARIANTS = {
     "hash_g2": [(ph, b, F.KIND_HASH, 2, 0) for ph in (0, 1) for b in range(2, 7)],
     "too_small_tables": [(ph, b, 0, 0, 4) for ph in (0, 1, 2) for b in range(2, 10)],
     "direct_g1": [(ph, 1, F.KIND_DIRECT, 1, 0) for ph in (0, 1, 2)],
+    "numeric_map_g1": [(2, b, F.KIND_MAP, 1, 0) for b in range(2, 5)],
+    "numeric_map_g4": [(2, b, F.KIND_MAP, 4, 0) for b in range(2, 10)],
+    "numeric_map_g64": [(2, b, F.KIND_MAP, 64, 0) for b in range(2, 10)],
 }
+
+
+@pytest.mark.parametrize("name", ["cfg1_uniform16_n4096", "cfg2_laplacian_g1024_it0",
+                                  "cfg3_rmat_s12", "cfg5_banded_small"])
+def test_position_map_repeated_numeric(gpu, name):
+    # The map is built on the first compute and drives every later one:
+    # redraw the values, recompute numeric-only, compare with the oracle.
+    A, B = cases.build(name)
+    choices = [(2, b, F.KIND_MAP, g, 0) for b in range(2, 10) for g in (2,)]
+    dA = rvsp.DeviceCsr.upload(A)
+    dB = dA if B is A else rvsp.DeviceCsr.upload(B)
+    d = rvsp.Descriptor(memory=F.MEMORY_DEVICE, choices=choices)
+    nnz = d.estimate(dA, dB)
+    assert any(p["kind"] == "lds_map" and p["rows"] > 0 for p in d.plan())
+    dC = rvsp.DeviceCsr(A.rows, B.cols, nnz)
+    Cm = dC.struct(nnz)
+    rng = np.random.Generator(np.random.PCG64(77))
+    square = B is A
+    for it in range(3):
+        if it:
+            A = A.with_values(rng.uniform(-1, 1, A.nnz).astype(np.float32))
+            dA.set_values(A.values)
+            if square:
+                B = A
+        F.check(d.compute(dA, dB, Cm))
+        same(dC.download(), oracle.spgemm(A, B))
+    d.close()
+    dC.free()
+    dA.free()
+    if dB is not dA:
+        dB.free()
 
 
 @pytest.mark.parametrize("variant", sorted(PLAN_VARIANTS))
