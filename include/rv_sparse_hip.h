@@ -109,6 +109,8 @@ typedef struct
     int32_t kind;
     int32_t group;
     int32_t cap;
+    int32_t variant; /* kernel variant of `kind` (RVSP_KIND_MAP: 1 stages A's
+                        metadata in LDS); taken together with a non-zero kind */
 } rvsp_bin_choice_t;
 
 /* Replaces the descriptor's override list (n == 0 clears it). Must precede

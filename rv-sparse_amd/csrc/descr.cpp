@@ -275,6 +275,7 @@ rvsp_status_t select_phase(rvsp_spgemm_descr* d, PhasePlan& p, int phase, const 
     bc.kind = c.kind;
     bc.group = c.group;
     bc.cap = c.cap;
+    bc.variant = c.variant;
     apply_choice(p.table, phase, c.bin, bc, p.table.rule.bound[c.bin]);
   }
   clamp_to_device(p.table);

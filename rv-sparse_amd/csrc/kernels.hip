@@ -1064,8 +1064,11 @@ hipError_t launch_build_map(int width, const int* rows, int nrows, const DevCsr&
 // Numeric pass driven by the position map. Same row/group structure,
 // segment mode and load pipeline as k_numeric_search, but LDS holds only the
 // accumulators (and, in segment mode, the segment's slice of the map), and
-// each product is one map entry, one B value and one LDS add.
-template <int G, typename T>
+// each product is one map entry, one B value and one LDS add. STAGE_A also
+// stages every A entry's scale and B-row bounds of the segment in LDS, which
+// removes the per-group (G-fold redundant) scattered loads of A and B's row
+// pointers at the cost of LDS and one more load round before the barrier.
+template <int G, typename T, bool STAGE_A>
 __global__ __launch_bounds__(kBlock) void k_numeric_map(
     const int* __restrict__ rows, int nrows, int cap, DevCsr A, DevCsr B,
     const int* __restrict__ c_rp, const long long* __restrict__ poff,
@@ -1073,9 +1076,13 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
   extern __shared__ int lds[];
   constexpr int kGroups = kBlock / G;
   constexpr int kMapCap = kGroups * kSegMapPerRow;  // map entries staged per segment
+  constexpr int kACap = kGroups * kSegAPerRow;      // A entries staged per segment
   constexpr int kU = 2;
   float* vals = reinterpret_cast<float*>(lds);
   T* mstage = reinterpret_cast<T*>(lds + kGroups * cap);
+  float* m_a = reinterpret_cast<float*>(lds + kGroups * cap + (kMapCap * (int)sizeof(T) + 3) / 4);
+  int* m_bs = reinterpret_cast<int*>(m_a + kACap);
+  int* m_be = m_bs + kACap;
 
   const int r0 = xcd_block(blockIdx.x, gridDim.x) * kGroups;
   const int nr = min(kGroups, nrows - r0);
@@ -1092,31 +1099,10 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
   const int a_s = active ? A.rp[row] : 0;
   const int a_e = active ? A.rp[row + 1] : 0;
 
-  // Load pipeline, primed before staging (see k_numeric_search).
-  int k1 = 0, k2 = 0, bs0 = 0, be0 = 0, bs1 = 0, be1 = 0;
-  float a0 = 0.0f, a1 = 0.0f;
-  if (a_s < a_e) {
-    const int k0 = A.ci[a_s];
-    a0 = A.val[a_s];
-    bs0 = B.rp[k0];
-    be0 = B.rp[k0 + 1];
-  }
-  if (a_s + 1 < a_e) {
-    k1 = A.ci[a_s + 1];
-    a1 = A.val[a_s + 1];
-    bs1 = B.rp[k1];
-    be1 = B.rp[k1 + 1];
-  }
-  if (a_s + 2 < a_e) k2 = A.ci[a_s + 2];
-  float v0[kU], v1[kU];
-#pragma unroll
-  for (int j = 0; j < kU; ++j) {
-    const int p = bs0 + l + j * G;
-    v0[j] = p < be0 ? B.val[p] : 0.0f;
-  }
-
   int base;
   int s0 = 0, s1 = 0;
+  int a_off = 0;          // subtracted from A positions to index staged metadata
+  bool a_staged = false;  // uniform
   const T* rm = map + mo;  // this row's map entries
   if (seg) {
     s0 = c_rp[first];
@@ -1124,15 +1110,55 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
     const long long m0 = poff[first];
     const long long m1 = poff[last + 1];
     const bool staged = m1 - m0 <= kMapCap;  // uniform
+    if (STAGE_A) {
+      a_off = A.rp[first];
+      a_staged = A.rp[last + 1] - a_off <= kACap;
+    }
     for (int i = threadIdx.x; i < s1 - s0; i += kBlock) vals[i] = 0.0f;
     if (staged)
       for (int i = threadIdx.x; i < (int)(m1 - m0); i += kBlock) mstage[i] = map[m0 + i];
+    if (a_staged) {
+      const int na = A.rp[last + 1] - a_off;
+      for (int i = threadIdx.x; i < na; i += kBlock) {
+        const int k = A.ci[a_off + i];
+        m_a[i] = A.val[a_off + i];
+        m_bs[i] = B.rp[k];
+        m_be[i] = B.rp[k + 1];
+      }
+    }
     __syncthreads();
     base = c_s - s0;
     if (staged) rm = mstage + (mo - m0);
   } else {
     base = g * cap;
     for (int j = l; j < n; j += G) vals[base + j] = 0.0f;
+  }
+
+  // Scale and B-row bounds of A entry e.
+  auto meta = [&](int e, float* a, int* bs, int* be) {
+    if (a_staged) {
+      *a = m_a[e - a_off];
+      *bs = m_bs[e - a_off];
+      *be = m_be[e - a_off];
+    } else {
+      const int k = A.ci[e];
+      *a = A.val[e];
+      *bs = B.rp[k];
+      *be = B.rp[k + 1];
+    }
+  };
+
+  // Load pipeline: entry ap's B values (kU per lane) in registers, entry
+  // ap+1's scale and bounds known, while entry ap is applied.
+  int bs0 = 0, be0 = 0, bs1 = 0, be1 = 0;
+  float a0 = 0.0f, a1 = 0.0f;
+  if (a_s < a_e) meta(a_s, &a0, &bs0, &be0);
+  if (a_s + 1 < a_e) meta(a_s + 1, &a1, &bs1, &be1);
+  float v0[kU], v1[kU];
+#pragma unroll
+  for (int j = 0; j < kU; ++j) {
+    const int p = bs0 + l + j * G;
+    v0[j] = p < be0 ? B.val[p] : 0.0f;
   }
 
   if (active) {
@@ -1144,11 +1170,9 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
         const int p = bs1 + l + j * G;
         v1[j] = p < be1 ? B.val[p] : 0.0f;
       }
-      const bool has2 = ap + 2 < a_e;
-      const int bs2 = has2 ? B.rp[k2] : 0;
-      const int be2 = has2 ? B.rp[k2 + 1] : 0;
-      const float a2 = has2 ? A.val[ap + 2] : 0.0f;
-      const int k3 = ap + 3 < a_e ? A.ci[ap + 3] : 0;
+      float a2 = 0.0f;
+      int bs2 = 0, be2 = 0;
+      if (ap + 2 < a_e) meta(ap + 2, &a2, &bs2, &be2);
       const int len = be0 - bs0;
 #pragma unroll
       for (int j = 0; j < kU; ++j) {
@@ -1171,7 +1195,6 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
       a1 = a2;
       bs1 = bs2;
       be1 = be2;
-      k2 = k3;
     }
   }
 
@@ -1190,12 +1213,15 @@ static hipError_t launch_numeric_map_g(const BinSpec& b, int width, const int* r
                                        hipStream_t s) {
   const int groups = kBlock / G;
   const int grid = (nrows + groups - 1) / groups;
-  if (width == 1)
-    hipLaunchKernelGGL((k_numeric_map<G, unsigned char>), dim3(grid), dim3(kBlock), b.lds_bytes,
-                       s, rows, nrows, b.cap, A, B, c_rp, poff, (const unsigned char*)map, c_val);
-  else
-    hipLaunchKernelGGL((k_numeric_map<G, unsigned short>), dim3(grid), dim3(kBlock), b.lds_bytes,
-                       s, rows, nrows, b.cap, A, B, c_rp, poff, (const unsigned short*)map, c_val);
+#define RVSP_MAP_LAUNCH(T, SA)                                                              \
+  hipLaunchKernelGGL((k_numeric_map<G, T, SA>), dim3(grid), dim3(kBlock), b.lds_bytes, s, rows, \
+                     nrows, b.cap, A, B, c_rp, poff, (const T*)map, c_val)
+  if (width == 1) {
+    if (b.variant) RVSP_MAP_LAUNCH(unsigned char, true); else RVSP_MAP_LAUNCH(unsigned char, false);
+  } else {
+    if (b.variant) RVSP_MAP_LAUNCH(unsigned short, true); else RVSP_MAP_LAUNCH(unsigned short, false);
+  }
+#undef RVSP_MAP_LAUNCH
   return hipGetLastError();
 }
 
@@ -1372,6 +1398,14 @@ hipError_t init_kernel_attributes(int lds_bytes) {
   RVSP_ALLOW(k_numeric_search<64>)
   RVSP_ALLOW((k_bitmap_row<false, false>)) RVSP_ALLOW((k_bitmap_row<false, true>))
   RVSP_ALLOW(k_numeric_rowblock<false>)
+#define RVSP_ALLOW_MAP(G)                                                     \
+  RVSP_ALLOW((k_numeric_map<G, unsigned char, false>))                         \
+  RVSP_ALLOW((k_numeric_map<G, unsigned char, true>))                          \
+  RVSP_ALLOW((k_numeric_map<G, unsigned short, false>))                        \
+  RVSP_ALLOW((k_numeric_map<G, unsigned short, true>))
+  RVSP_ALLOW_MAP(1) RVSP_ALLOW_MAP(2) RVSP_ALLOW_MAP(4) RVSP_ALLOW_MAP(8)
+  RVSP_ALLOW_MAP(16) RVSP_ALLOW_MAP(32) RVSP_ALLOW_MAP(64)
+#undef RVSP_ALLOW_MAP
 #undef RVSP_ALLOW
   return e;
 }

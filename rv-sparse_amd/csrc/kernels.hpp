@@ -71,6 +71,7 @@ struct BinSpec {
   int cap = 0;       // per-row LDS capacity: table slots (HASH), columns
                      // (SEARCH/BLOCK), bitmap bits (BITMAP), span (DENSE)
   int lds_bytes = 0; // dynamic LDS per workgroup
+  int variant = 0;   // kernel variant of the kind (MAP: 1 = A metadata staged in LDS)
 };
 
 // Scratch for the partition step.

@@ -43,7 +43,7 @@ const char* kind_name(int kind) {
   }
 }
 
-int lds_bytes_for(int phase, int kind, int group, int cap) {
+int lds_bytes_for(int phase, int kind, int group, int cap, int variant) {
   const bool sub_wave = pow2(group) && group <= kWave;
   long long bytes = -1;
   switch (kind) {
@@ -72,6 +72,8 @@ int lds_bytes_for(int phase, int kind, int group, int cap) {
       if (phase != PHASE_NUMERIC || !sub_wave || cap < 1 || cap > 65536) break;
       // Accumulators plus the staged slice of the map (up to 2-byte entries).
       bytes = (long long)(kBlock / group) * cap * 4 + ((kBlock / group) * kSegMapPerRow * 2 + 3) / 4 * 4;
+      if (variant == 1) bytes += (long long)(kBlock / group) * kSegAPerRow * 12;
+      else if (variant != 0) bytes = -1;
       break;
     case KIND_BITMAP:
       if (phase == PHASE_NUMERIC || cap < 32 || (cap & 31)) break;
@@ -143,7 +145,10 @@ Strategy fixed_strategy() {
 bool apply_choice(PhaseTable& t, int phase, int b, const BinChoice& c,
                   int max_metric) {
   BinSpec cand = t.spec[b];
-  if (c.kind) cand.kind = c.kind;
+  if (c.kind) {
+    cand.kind = c.kind;
+    cand.variant = c.variant;
+  }
   if (c.group) cand.group = c.group;
   if (c.cap) cand.cap = c.cap;
   bool ok = true;
@@ -178,7 +183,7 @@ bool apply_choice(PhaseTable& t, int phase, int b, const BinChoice& c,
     }
   }
   if (ok) {
-    cand.lds_bytes = lds_bytes_for(phase, cand.kind, cand.group, cand.cap);
+    cand.lds_bytes = lds_bytes_for(phase, cand.kind, cand.group, cand.cap, cand.variant);
     ok = cand.lds_bytes >= 0;
   }
   if (ok) {
@@ -190,6 +195,7 @@ bool apply_choice(PhaseTable& t, int phase, int b, const BinChoice& c,
     t.spec[b].group = kBlock;
     t.spec[b].cap = 0;
     t.spec[b].lds_bytes = 0;
+    t.spec[b].variant = 0;
   }
   return false;
 }

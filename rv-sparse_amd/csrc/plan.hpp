@@ -22,7 +22,7 @@ struct Strategy {
 
 // LDS bytes a workgroup needs for a bin of this kind/group/capacity in the
 // given phase; -1 if the combination is not launchable.
-int lds_bytes_for(int phase, int kind, int group, int cap);
+int lds_bytes_for(int phase, int kind, int group, int cap, int variant = 0);
 
 // The plain fixed-strategy table (RVSP_SPGEMM_ALGO_DEFAULT / HIP_FIXED).
 Strategy fixed_strategy();
@@ -33,6 +33,7 @@ struct BinChoice {
   int kind = 0;
   int group = 0;
   int cap = 0;
+  int variant = 0;  // applied together with `kind`
 };
 
 // Applies `choice` to bin b of `t` if the result is launchable and holds the

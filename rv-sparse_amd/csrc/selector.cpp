@@ -99,13 +99,15 @@ int load_locked(const std::string& path) {
   int n = 0;
   while (fgets(line, sizeof line, f)) {
     if (line[0] == '#') continue;
-    int ph, b, lc, kind, group, cap;
+    int ph, b, lc, kind, group, cap, variant = 0;
     float us = 0.f;
-    if (sscanf(line, "%d %d %d %d %d %d %f", &ph, &b, &lc, &kind, &group, &cap, &us) >= 6) {
+    if (sscanf(line, "%d %d %d %d %d %d %f %d", &ph, &b, &lc, &kind, &group, &cap, &us,
+               &variant) >= 6) {
       TuneEntry e;
       e.choice.kind = kind;
       e.choice.group = group;
       e.choice.cap = cap;
+      e.choice.variant = variant;
       e.us = us;
       g_table[TuneKey(ph, b, lc)] = e;
       ++n;
@@ -177,6 +179,9 @@ std::vector<BinChoice> candidates(const PhaseTable& t, int phase, int b, bool ma
       out.push_back(c);
       if (map_ok && (cur.kind == KIND_SEARCH || cur.kind == KIND_MAP)) {
         c.kind = KIND_MAP;
+        c.variant = 0;
+        out.push_back(c);
+        c.variant = 1;
         out.push_back(c);
       }
     }
@@ -266,6 +271,9 @@ rvsp_status_t autotune_plan(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr&
           break;
         }
         const float us = ms * 1000.f / reps;
+        if (getenv("RVSP_TUNE_VERBOSE"))
+          fprintf(stderr, "rvsp tune: phase %d bin %d rows %d  %-12s v%d G=%-3d cap=%-6d %9.2f us\n",
+                  ph, b, n, kind_name(spec.kind), spec.variant, spec.group, spec.cap, us);
         if (!have || us < best_us) {
           have = true;
           best_us = us;
@@ -278,6 +286,7 @@ rvsp_status_t autotune_plan(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr&
       te.choice.kind = best.kind;
       te.choice.group = best.group;
       te.choice.cap = best.cap;
+      te.choice.variant = best.variant;
       te.us = best_us;
       std::lock_guard<std::mutex> lock(g_table_mu);
       ensure_loaded_locked();
@@ -298,11 +307,11 @@ int tuning_table_save(const char* path) {
   ensure_loaded_locked();
   FILE* f = fopen(path ? path : default_table_path().c_str(), "w");
   if (!f) return -1;
-  fprintf(f, "# rv-sparse gfx950 tuning table: phase bin len_class kind group cap us\n");
+  fprintf(f, "# rv-sparse gfx950 tuning table: phase bin len_class kind group cap us variant\n");
   for (const auto& kv : g_table)
-    fprintf(f, "%d %d %d %d %d %d %.3f\n", std::get<0>(kv.first), std::get<1>(kv.first),
+    fprintf(f, "%d %d %d %d %d %d %.3f %d\n", std::get<0>(kv.first), std::get<1>(kv.first),
             std::get<2>(kv.first), kv.second.choice.kind, kv.second.choice.group,
-            kv.second.choice.cap, kv.second.us);
+            kv.second.choice.cap, kv.second.us, kv.second.choice.variant);
   fclose(f);
   return (int)g_table.size();
 }

@@ -63,7 +63,7 @@ class BinInfo(C.Structure):
 
 class BinChoice(C.Structure):
     _fields_ = [("phase", C.c_int32), ("bin", C.c_int32), ("kind", C.c_int32),
-                ("group", C.c_int32), ("cap", C.c_int32)]
+                ("group", C.c_int32), ("cap", C.c_int32), ("variant", C.c_int32)]
 
 
 P = C.c_void_p

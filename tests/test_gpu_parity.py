@@ -104,6 +104,7 @@ PLAN_VARIANTS = {
     "numeric_map_g1": [(2, b, F.KIND_MAP, 1, 0) for b in range(2, 5)],
     "numeric_map_g4": [(2, b, F.KIND_MAP, 4, 0) for b in range(2, 10)],
     "numeric_map_g64": [(2, b, F.KIND_MAP, 64, 0) for b in range(2, 10)],
+    "numeric_map_staged_a": [(2, b, F.KIND_MAP, 2, 0, 1) for b in range(2, 10)],
 }
 
 
@@ -113,7 +114,8 @@ def test_position_map_repeated_numeric(gpu, name):
     # The map is built on the first compute and drives every later one:
     # redraw the values, recompute numeric-only, compare with the oracle.
     A, B = cases.build(name)
-    choices = [(2, b, F.KIND_MAP, g, 0) for b in range(2, 10) for g in (2,)]
+    choices = [(2, b, F.KIND_MAP, 2, 0, v) for b in range(2, 10) for v in (0, 1)][1::2] \
+        if name.startswith("cfg2") else [(2, b, F.KIND_MAP, 2, 0) for b in range(2, 10)]
     dA = rvsp.DeviceCsr.upload(A)
     dB = dA if B is A else rvsp.DeviceCsr.upload(B)
     d = rvsp.Descriptor(memory=F.MEMORY_DEVICE, choices=choices)
