@@ -1064,11 +1064,12 @@ hipError_t launch_build_map(int width, const int* rows, int nrows, const DevCsr&
 // Numeric pass driven by the position map. Same row/group structure,
 // segment mode and load pipeline as k_numeric_search, but LDS holds only the
 // accumulators (and, in segment mode, the segment's slice of the map), and
-// each product is one map entry, one B value and one LDS add. STAGE_A also
+// each product is one map entry, one B value and one LDS add. VAR 1 also
 // stages every A entry's scale and B-row bounds of the segment in LDS, which
 // removes the per-group (G-fold redundant) scattered loads of A and B's row
-// pointers at the cost of LDS and one more load round before the barrier.
-template <int G, typename T, bool STAGE_A>
+// pointers at the cost of LDS and one more load round before the barrier
+// (VAR 1). VAR 2 instead fetches A entries in chunks (see below).
+template <int G, typename T, int VAR>
 __global__ __launch_bounds__(kBlock) void k_numeric_map(
     const int* __restrict__ rows, int nrows, int cap, DevCsr A, DevCsr B,
     const int* __restrict__ c_rp, const long long* __restrict__ poff,
@@ -1110,7 +1111,7 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
     const long long m0 = poff[first];
     const long long m1 = poff[last + 1];
     const bool staged = m1 - m0 <= kMapCap;  // uniform
-    if (STAGE_A) {
+    if (VAR == 1) {
       a_off = A.rp[first];
       a_staged = A.rp[last + 1] - a_off <= kACap;
     }
@@ -1148,6 +1149,57 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
     }
   };
 
+  if constexpr (VAR == 2) {
+    // Chunked loads: for up to kChunk A entries at once, their columns and
+    // scales, then their B-row bounds, then the first kU products of each
+    // are in flight together (three dependent rounds per chunk instead of
+    // one or more per entry); the entries are then applied in order.
+    constexpr int kChunk = 6;
+    if (active) {
+      float* rv = vals + base;
+      int t0 = 0;
+      for (int c0 = a_s; c0 < a_e; c0 += kChunk) {
+        int kk[kChunk], bb[kChunk], ee[kChunk];
+        float aa[kChunk], vv[kChunk][kU];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          const bool h = c0 + u < a_e;
+          kk[u] = h ? A.ci[c0 + u] : 0;
+          aa[u] = h ? A.val[c0 + u] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          const bool h = c0 + u < a_e;
+          bb[u] = h ? B.rp[kk[u]] : 0;
+          ee[u] = h ? B.rp[kk[u] + 1] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u)
+#pragma unroll
+          for (int j = 0; j < kU; ++j) {
+            const int p = bb[u] + l + j * G;
+            vv[u][j] = p < ee[u] ? B.val[p] : 0.0f;
+          }
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          const int len = ee[u] - bb[u];  // 0 past the row's last entry
+#pragma unroll
+          for (int j = 0; j < kU; ++j) {
+            const int t = l + j * G;
+            if (t < len) {
+              const int pos = rm[t0 + t];
+              rv[pos] = acc_step(rv[pos], aa[u], vv[u][j]);
+            }
+          }
+          for (int t = kU * G + l; t < len; t += G) {
+            const int pos = rm[t0 + t];
+            rv[pos] = acc_step(rv[pos], aa[u], B.val[bb[u] + t]);
+          }
+          t0 += len;
+        }
+      }
+    }
+  } else {
   // Load pipeline: entry ap's B values (kU per lane) in registers, entry
   // ap+1's scale and bounds known, while entry ap is applied.
   int bs0 = 0, be0 = 0, bs1 = 0, be1 = 0;
@@ -1197,6 +1249,7 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
       be1 = be2;
     }
   }
+  }  // VAR != 2
 
   if (seg) {
     __syncthreads();
@@ -1216,10 +1269,15 @@ static hipError_t launch_numeric_map_g(const BinSpec& b, int width, const int* r
 #define RVSP_MAP_LAUNCH(T, SA)                                                              \
   hipLaunchKernelGGL((k_numeric_map<G, T, SA>), dim3(grid), dim3(kBlock), b.lds_bytes, s, rows, \
                      nrows, b.cap, A, B, c_rp, poff, (const T*)map, c_val)
+  if (b.variant < 0 || b.variant > 2) return hipErrorInvalidValue;
   if (width == 1) {
-    if (b.variant) RVSP_MAP_LAUNCH(unsigned char, true); else RVSP_MAP_LAUNCH(unsigned char, false);
+    if (b.variant == 0) RVSP_MAP_LAUNCH(unsigned char, 0);
+    else if (b.variant == 1) RVSP_MAP_LAUNCH(unsigned char, 1);
+    else RVSP_MAP_LAUNCH(unsigned char, 2);
   } else {
-    if (b.variant) RVSP_MAP_LAUNCH(unsigned short, true); else RVSP_MAP_LAUNCH(unsigned short, false);
+    if (b.variant == 0) RVSP_MAP_LAUNCH(unsigned short, 0);
+    else if (b.variant == 1) RVSP_MAP_LAUNCH(unsigned short, 1);
+    else RVSP_MAP_LAUNCH(unsigned short, 2);
   }
 #undef RVSP_MAP_LAUNCH
   return hipGetLastError();
@@ -1399,10 +1457,12 @@ hipError_t init_kernel_attributes(int lds_bytes) {
   RVSP_ALLOW((k_bitmap_row<false, false>)) RVSP_ALLOW((k_bitmap_row<false, true>))
   RVSP_ALLOW(k_numeric_rowblock<false>)
 #define RVSP_ALLOW_MAP(G)                                                     \
-  RVSP_ALLOW((k_numeric_map<G, unsigned char, false>))                         \
-  RVSP_ALLOW((k_numeric_map<G, unsigned char, true>))                          \
-  RVSP_ALLOW((k_numeric_map<G, unsigned short, false>))                        \
-  RVSP_ALLOW((k_numeric_map<G, unsigned short, true>))
+  RVSP_ALLOW((k_numeric_map<G, unsigned char, 0>))                             \
+  RVSP_ALLOW((k_numeric_map<G, unsigned char, 1>))                             \
+  RVSP_ALLOW((k_numeric_map<G, unsigned char, 2>))                             \
+  RVSP_ALLOW((k_numeric_map<G, unsigned short, 0>))                            \
+  RVSP_ALLOW((k_numeric_map<G, unsigned short, 1>))                            \
+  RVSP_ALLOW((k_numeric_map<G, unsigned short, 2>))
   RVSP_ALLOW_MAP(1) RVSP_ALLOW_MAP(2) RVSP_ALLOW_MAP(4) RVSP_ALLOW_MAP(8)
   RVSP_ALLOW_MAP(16) RVSP_ALLOW_MAP(32) RVSP_ALLOW_MAP(64)
 #undef RVSP_ALLOW_MAP

@@ -73,7 +73,7 @@ int lds_bytes_for(int phase, int kind, int group, int cap, int variant) {
       // Accumulators plus the staged slice of the map (up to 2-byte entries).
       bytes = (long long)(kBlock / group) * cap * 4 + ((kBlock / group) * kSegMapPerRow * 2 + 3) / 4 * 4;
       if (variant == 1) bytes += (long long)(kBlock / group) * kSegAPerRow * 12;
-      else if (variant != 0) bytes = -1;
+      else if (variant != 0 && variant != 2) bytes = -1;
       break;
     case KIND_BITMAP:
       if (phase == PHASE_NUMERIC || cap < 32 || (cap & 31)) break;

@@ -183,6 +183,8 @@ std::vector<BinChoice> candidates(const PhaseTable& t, int phase, int b, bool ma
         out.push_back(c);
         c.variant = 1;
         out.push_back(c);
+        c.variant = 2;
+        out.push_back(c);
       }
     }
   }

@@ -105,6 +105,7 @@ PLAN_VARIANTS = {
     "numeric_map_g4": [(2, b, F.KIND_MAP, 4, 0) for b in range(2, 10)],
     "numeric_map_g64": [(2, b, F.KIND_MAP, 64, 0) for b in range(2, 10)],
     "numeric_map_staged_a": [(2, b, F.KIND_MAP, 2, 0, 1) for b in range(2, 10)],
+    "numeric_map_chunked": [(2, b, F.KIND_MAP, 1, 0, 2) for b in range(2, 10)],
 }
 
 
