@@ -11,6 +11,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -165,51 +166,6 @@ hipError_t launch_bin(rvsp_spgemm_descr* d, int phase, const BinSpec& spec, cons
   }
 }
 
-std::vector<BinChoice> candidates(const PhaseTable& t, int phase, int b, bool map_ok) {
-  std::vector<BinChoice> out;
-  const BinSpec& cur = t.spec[b];
-  const int groups[] = {1, 2, 4, 8, 16, 32, 64};
-  if (cur.kind == KIND_DIRECT || cur.kind == KIND_HASH || cur.kind == KIND_SEARCH ||
-      cur.kind == KIND_MAP) {
-    for (int g : groups) {
-      BinChoice c;
-      c.kind = cur.kind == KIND_MAP ? KIND_SEARCH : cur.kind;
-      c.group = g;
-      c.cap = cur.cap;
-      out.push_back(c);
-      if (map_ok && (cur.kind == KIND_SEARCH || cur.kind == KIND_MAP)) {
-        c.kind = KIND_MAP;
-        c.variant = 0;
-        out.push_back(c);
-        c.variant = 1;
-        out.push_back(c);
-        c.variant = 2;
-        out.push_back(c);
-      }
-    }
-  }
-  if (phase == PHASE_NUMERIC && (cur.kind == KIND_SEARCH || cur.kind == KIND_BLOCK)) {
-    BinChoice c;
-    c.kind = KIND_BLOCK;
-    c.group = kBlock;
-    c.cap = t.rule.bound[b] <= 4096 ? 4096 : 16384;
-    out.push_back(c);
-  }
-  if (cur.kind == KIND_BLOCK || cur.kind == KIND_BITMAP || cur.kind == KIND_GLOBAL) {
-    BinChoice c;
-    c.kind = cur.kind;
-    c.group = kBlock;
-    c.cap = cur.cap;
-    out.push_back(c);
-    if (cur.kind != KIND_GLOBAL) {
-      BinChoice g;
-      g.kind = KIND_GLOBAL;
-      out.push_back(g);
-    }
-  }
-  return out;
-}
-
 }  // namespace
 
 rvsp_status_t autotune_plan(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
@@ -252,9 +208,22 @@ rvsp_status_t autotune_plan(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr&
       BinSpec best = p.table.spec[b];
       bool have = false;
       const bool map_ok = ph == PHASE_NUMERIC && ((d->map_bins >> b) & 1u) && d->map_valid;
-      for (const BinChoice& c : candidates(p.table, ph, b, map_ok)) {
+      const int max_metric = (int)std::min(feats[b].max_metric, 2147483647.0);
+      // Every candidate class is timed and recorded as one training sample
+      // of the learned selector.
+      rvsp_learned_sample_t smp;
+      memset(&smp, 0, sizeof smp);
+      smp.phase = ph;
+      smp.bin = b;
+      smp.fixed_class = fixed_class(ph, b);
+      for (int c = 0; c < RVSP_MLP_CLASSES; ++c) smp.us[c] = -1.f;
+      bin_feature_vector(ph, b, p.table.rule.bound[b], feats[b], smp.features);
+      const uint64_t cands = candidate_classes(ph, b, map_ok);
+      for (int cls = 0; cls < RVSP_MLP_CLASSES; ++cls) {
+        if (!((cands >> cls) & 1ull)) continue;
         PhaseTable trial = p.table;
-        if (!apply_choice(trial, ph, b, c, (int)feats[b].max_metric)) continue;
+        if (!apply_choice(trial, ph, b, class_choice(cls, ph, b, max_metric), max_metric))
+          continue;
         const BinSpec& spec = trial.spec[b];
         if (spec.lds_bytes > lds_limit()) continue;
         const GlobalSlots& gs = ph == PHASE_NUMERIC ? d->gs_dense : d->gs_bits;
@@ -273,6 +242,8 @@ rvsp_status_t autotune_plan(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr&
           break;
         }
         const float us = ms * 1000.f / reps;
+        smp.us[cls] = us;
+        ++smp.n_candidates;
         if (getenv("RVSP_TUNE_VERBOSE"))
           fprintf(stderr, "rvsp tune: phase %d bin %d rows %d  %-12s v%d G=%-3d cap=%-6d %9.2f us\n",
                   ph, b, n, kind_name(spec.kind), spec.variant, spec.group, spec.cap, us);
@@ -283,6 +254,7 @@ rvsp_status_t autotune_plan(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr&
         }
       }
       if (status != RVSP_SUCCESS || !have) break;
+      record_sample(smp);
       p.table.spec[b] = best;
       TuneEntry te;
       te.choice.kind = best.kind;

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "descr_impl.hpp"
+#include "rv_sparse_learned.h"
 
 namespace rvsp {
 
@@ -27,6 +28,21 @@ void select_tuned(PhaseTable& t, int phase, const std::vector<BinFeatures>& f);
 // HIP_LEARNED: the MLP selector's choice for every non-empty bin.
 rvsp_status_t select_learned(PhaseTable& t, int phase, const std::vector<BinFeatures>& f,
                              hipStream_t s);
+
+// Output classes of the selector network (rv_sparse_learned.h). A class is a
+// kernel shape relative to the fixed table's bin: its kind, lanes per row and
+// a multiple of the bin's base table size.
+// The class the fixed table uses for bin b (-1 for bin 0).
+int fixed_class(int phase, int b);
+// Classes that may serve bin b (bit c = class c); map_ok admits the MAP
+// classes of a numeric bin.
+uint64_t candidate_classes(int phase, int b, bool map_ok);
+// The choice class `cls` stands for in bin b, given the bin's largest row.
+BinChoice class_choice(int cls, int phase, int b, int max_metric);
+// The network's raw input for bin b.
+void bin_feature_vector(int phase, int b, int bound, const BinFeatures& f, float* out);
+// Appends a timed bin to the process-wide sample store.
+void record_sample(const rvsp_learned_sample_t& s);
 
 // Times every launchable shape of every non-empty bin of the descriptor's
 // plan and keeps the fastest (also recorded in the tuning table). c_ci and

@@ -66,9 +66,29 @@ class BinChoice(C.Structure):
                 ("group", C.c_int32), ("cap", C.c_int32), ("variant", C.c_int32)]
 
 
+# include/rv_sparse_learned.h
+MLP_FEATURES, MLP_HIDDEN, MLP_CLASSES = 16, 64, 52
+MLP_PARAMS = (MLP_HIDDEN * MLP_FEATURES + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN
+              + MLP_CLASSES * MLP_HIDDEN + MLP_CLASSES)
+
+
+class LearnedClass(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("group", C.c_int32), ("cap_mult", C.c_int32),
+                ("variant", C.c_int32)]
+
+
+class LearnedSample(C.Structure):
+    _fields_ = [("phase", C.c_int32), ("bin", C.c_int32), ("fixed_class", C.c_int32),
+                ("n_candidates", C.c_int32), ("features", C.c_float * MLP_FEATURES),
+                ("us", C.c_float * MLP_CLASSES)]
+
+
 P = C.c_void_p
 PM = C.POINTER(CsrMatrix)
 I32, I64, SZ = C.c_int32, C.c_int64, C.c_size_t
+FP = C.POINTER(C.c_float)
+U64P = C.POINTER(C.c_uint64)
+I32P = C.POINTER(C.c_int32)
 
 # name -> (restype, argtypes). Exactly the functions the headers declare.
 PROTOTYPES = {
@@ -110,6 +130,21 @@ PROTOTYPES = {
     "rvsp_hip_memcpy": (C.c_int, [P, P, SZ, C.c_int]),
     "rvsp_hip_synchronize": (C.c_int, []),
     "rvsp_hip_device_count": (I32, []),
+    # include/rv_sparse_learned.h
+    "rvsp_learned_class_info": (C.c_int, [I32, C.POINTER(LearnedClass)]),
+    "rvsp_learned_sample_count": (I32, []),
+    "rvsp_learned_sample_get": (C.c_int, [I32, C.POINTER(LearnedSample)]),
+    "rvsp_learned_sample_add": (C.c_int, [C.POINTER(LearnedSample)]),
+    "rvsp_learned_sample_clear": (None, []),
+    "rvsp_learned_fit": (C.c_int, [I32, I32, I32, C.c_float, C.c_uint64, FP]),
+    "rvsp_learned_model_save": (I32, [C.c_char_p]),
+    "rvsp_learned_model_load": (I32, [C.c_char_p]),
+    "rvsp_learned_model_clear": (None, []),
+    "rvsp_learned_model_ready": (I32, []),
+    "rvsp_learned_predict": (C.c_int, [FP, U64P, I32, I32P]),
+    "rvsp_mlp_forward": (C.c_int, [FP, FP, U64P, I32, FP, I32P]),
+    "rvsp_mlp_gradient": (C.c_int, [FP, FP, FP, U64P, FP, I32, FP, FP]),
+    "rvsp_mlp_train": (C.c_int, [FP, FP, FP, U64P, FP, I32, I32, C.c_float, FP]),
 }
 
 _lib = None

@@ -20,7 +20,9 @@ L = F.lib()
 
 def declared_functions():
     names = set()
-    for h in ("rv_sparse.h", "rvsp_spgemm.h", "rv_sparse_hip.h"):
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         names |= set(re.findall(r"\b(rvsp_\w+)\s*\(", text))
