@@ -142,6 +142,46 @@ int32_t rvsp_tuning_table_load(const char *path);
 void rvsp_tuning_table_clear(void);
 int32_t rvsp_tuning_table_size(void);
 
+/*
+ * Batched products: C[p] = A[p] * B[p] for p < count, all pairs served by
+ * one set of launches per phase instead of one per pair. A, B and C are
+ * arrays of `count` matrices; RVSP_MEMORY_DEVICE descriptors only
+ * (RVSP_ERROR_INVALID_ARGUMENT otherwise). Every pair's C is bit-identical to
+ * what rvsp_spgemm_compute gives for that pair alone, and every check of the
+ * single-product calls applies per pair.
+ *
+ * The pairs are stacked into one block-diagonal product, cut into groups of
+ * consecutive pairs whose product count fits int32 row pointers. Pairs
+ * passing the same B arrays share one stacked copy of B. When the C arrays
+ * of the pairs of a group lie back to back in memory (C[p+1].col_idx ==
+ * C[p].col_idx + c_nnz[p], likewise values), they are written in place;
+ * otherwise through one segmented copy.
+ *
+ * Repeated batch computes with the same arrays are numeric-only (like
+ * rvsp_spgemm_compute's reuse of C.col_idx); from the second such compute on
+ * they replay one captured HIP graph.
+ */
+rvsp_status_t rvsp_spgemm_batch_work_estimation(rvsp_spgemm_descr_t descr,
+                                                int32_t count,
+                                                const rvsp_csr_matrix_t *A,
+                                                const rvsp_csr_matrix_t *B,
+                                                int32_t *c_nnz);
+rvsp_status_t rvsp_spgemm_batch_compute(rvsp_spgemm_descr_t descr,
+                                        int32_t count,
+                                        const rvsp_csr_matrix_t *A,
+                                        const rvsp_csr_matrix_t *B,
+                                        rvsp_csr_matrix_t *C);
+/* Groups formed by the last batch estimation (-1 before one). */
+int32_t rvsp_spgemm_batch_groups(rvsp_spgemm_descr_t descr);
+/* Batch computes served by a graph launch since the last batch estimation. */
+int32_t rvsp_spgemm_batch_graph_launches(rvsp_spgemm_descr_t descr);
+/* on == 0: repeated batch computes launch eagerly instead of from a graph. */
+rvsp_status_t rvsp_spgemm_batch_set_graph(rvsp_spgemm_descr_t descr, int on);
+/* Caps the product count of a group (default and maximum INT32_MAX; 0
+ * restores the default). Must precede the batch estimation. */
+rvsp_status_t rvsp_spgemm_batch_set_group_limit(rvsp_spgemm_descr_t descr,
+                                                int64_t max_products);
+
 /* Device memory helpers for callers without a HIP binding of their own
  * (for example ctypes). kind: 1 host->device, 2 device->host,
  * 3 device->device. */

@@ -1,0 +1,601 @@
+// batch.cpp — batched SpGEMM: C[p] = A[p] * B[p] for many independent pairs
+// in one set of launches per phase, repeated numeric-only computes replayed
+// from a captured HIP graph. No counterpart in the reference; every pair's
+// result is exactly what rvsp_spgemm_compute gives for that pair alone.
+//
+// The batch is packed into one block-diagonal product. The distinct B
+// operands are stacked (rows concatenated, column indices kept local), each
+// A is stacked with its column indices moved to the rows of its B in the
+// stack, and the stacked A times the stacked B is the stacked C: a row of
+// pair p reaches only rows of B[p], and its columns stay pair-local. Every
+// kernel of the single-product path then serves all pairs at once, and row
+// order, accumulation order and output bits are those of each pair alone.
+//
+// The output row pointers are int32, so the stack is cut into groups of
+// consecutive pairs whose product count (an upper bound of their nnz) fits;
+// each group is one inner descriptor over a row range of the stacked A.
+// When the caller's C arrays of a group sit back to back in memory, the group
+// writes them in place; otherwise it writes its own buffer and one segmented
+// copy moves every pair's part out.
+#include <string.h>
+
+#include <algorithm>
+#include <climits>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "batch.hpp"
+#include "common.hpp"
+#include "descr_impl.hpp"
+
+namespace rvsp {
+
+struct BatchGroup {
+  int p0 = 0, p1 = 0;         // pairs [p0, p1)
+  long long r0 = 0, r1 = 0;   // rows of the stacked A
+  rvsp_spgemm_descr* inner = nullptr;
+  int32_t c_nnz = 0;
+  std::vector<long long> off;  // nnz offset of pair p0 + i within the group
+  DevBuf c_rp, c_ci, c_val;
+  bool in_place = false;       // C arrays of the last compute were back to back
+};
+
+// A segment table uploaded to the device, with its host copy kept alive
+// until the next rebuild (asynchronous uploads read it).
+struct SegTable {
+  std::vector<SegCopy> segs;
+  std::vector<SegChunk> chunks;
+  DevBuf d_segs, d_chunks;
+  void clear() {
+    segs.clear();
+    chunks.clear();
+  }
+  void add(const int* src, int* dst, long long len, int add, int rebase) {
+    if (len <= 0) return;
+    SegCopy s;
+    s.src = src;
+    s.dst = dst;
+    s.len = len;
+    s.add = add;
+    s.rebase = rebase;
+    const int id = (int)segs.size();
+    segs.push_back(s);
+    for (long long b = 0; b < len; b += kSegChunk) {
+      SegChunk c;
+      c.seg = id;
+      c.pad = 0;
+      c.begin = b;
+      chunks.push_back(c);
+    }
+  }
+  rvsp_status_t upload(hipStream_t s) {
+    if (segs.empty()) return RVSP_SUCCESS;
+    RVSP_TRY(ensure(d_segs, segs.size() * sizeof(SegCopy)));
+    RVSP_TRY(ensure(d_chunks, chunks.size() * sizeof(SegChunk)));
+    RVSP_HIP_TRY(hipMemcpyAsync(d_segs.p, segs.data(), segs.size() * sizeof(SegCopy),
+                                hipMemcpyHostToDevice, s));
+    RVSP_HIP_TRY(hipMemcpyAsync(d_chunks.p, chunks.data(), chunks.size() * sizeof(SegChunk),
+                                hipMemcpyHostToDevice, s));
+    return RVSP_SUCCESS;
+  }
+  hipError_t launch(hipStream_t s) const {
+    return launch_seg_copy((const SegCopy*)d_segs.p, (const SegChunk*)d_chunks.p,
+                           (int)chunks.size(), s);
+  }
+  void release_all() {
+    release(d_segs);
+    release(d_chunks);
+  }
+};
+
+struct BatchState {
+  bool estimated = false;
+  int count = 0;
+  std::vector<rvsp_csr_matrix_t> a, b;  // operands as estimated
+  std::vector<int> b_of;                // stacked-B index of every pair
+  std::vector<int> b_pair;              // first pair of every stacked B
+  std::vector<long long> a_row_off, a_nnz_off, b_row_off, b_nnz_off;
+  int b_cols = 1;
+  DevBuf a_rp, a_ci, a_val, b_rp, b_ci, b_val;
+  DevBuf ops, gather_idx, gather_out;
+  std::vector<BatchGroup> groups;
+  std::vector<int32_t> c_nnz;
+  SegTable t_struct, t_vals, t_out_full, t_out_vals;
+  std::vector<const void*> struct_key, vals_key, out_key, last_key;
+  bool out_filled = false;  // t_out_full already ran for out_key
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<const void*> graph_key;
+  bool graph_failed = false;
+  int graph_launches = 0;
+
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    graph_key.clear();
+  }
+  void drop_groups() {
+    for (BatchGroup& g : groups) {
+      rvsp_spgemm_descr_destroy(g.inner);
+      release(g.c_rp);
+      release(g.c_ci);
+      release(g.c_val);
+    }
+    groups.clear();
+  }
+  ~BatchState() {
+    drop_graph();
+    drop_groups();
+    DevBuf* bufs[] = {&a_rp, &a_ci, &a_val, &b_rp, &b_ci, &b_val, &ops, &gather_idx,
+                      &gather_out};
+    for (DevBuf* x : bufs) release(*x);
+    t_struct.release_all();
+    t_vals.release_all();
+    t_out_full.release_all();
+    t_out_vals.release_all();
+  }
+};
+
+void destroy_batch(rvsp_spgemm_descr* d) {
+  delete d->batch;
+  d->batch = nullptr;
+}
+
+void invalidate_batch(rvsp_spgemm_descr* d) {
+  if (!d->batch) return;
+  d->batch->drop_graph();
+  d->batch->out_filled = false;
+  d->batch->last_key.clear();
+  for (BatchGroup& g : d->batch->groups) rvsp_spgemm_invalidate_structure(g.inner);
+}
+
+namespace {
+
+bool same_matrix(const rvsp_csr_matrix_t& x, const rvsp_csr_matrix_t& y) {
+  return x.rows == y.rows && x.cols == y.cols && x.nnz == y.nnz && x.row_ptr == y.row_ptr &&
+         x.col_idx == y.col_idx && x.values == y.values;
+}
+
+rvsp_csr_matrix_t view(int rows, int cols, long long nnz, const void* rp, const void* ci,
+                       const void* val) {
+  rvsp_csr_matrix_t m;
+  memset(&m, 0, sizeof m);
+  m.rows = rows;
+  m.cols = cols;
+  m.nnz = (int32_t)nnz;
+  m.row_ptr = (int32_t*)rp;
+  m.col_idx = (int32_t*)ci;
+  m.values = (void*)val;
+  m.dtype = RVSP_DTYPE_FP32;
+  m.format = RVSP_FORMAT_CSR;
+  m.owns_data = 0;
+  return m;
+}
+
+// Stacked row pointers and (shifted) column indices of every A and B.
+void build_struct_table(BatchState& bs, const rvsp_csr_matrix_t* A, const rvsp_csr_matrix_t* B) {
+  SegTable& t = bs.t_struct;
+  t.clear();
+  const int n = bs.count;
+  for (int p = 0; p < n; ++p) {
+    const long long rows = A[p].rows + (p == n - 1 ? 1 : 0);
+    t.add(A[p].row_ptr, (int*)bs.a_rp.p + bs.a_row_off[p], rows, (int)bs.a_nnz_off[p], 1);
+    t.add(A[p].col_idx, (int*)bs.a_ci.p + bs.a_nnz_off[p], A[p].nnz,
+          (int)bs.b_row_off[bs.b_of[p]], 0);
+  }
+  const int nb = (int)bs.b_pair.size();
+  for (int j = 0; j < nb; ++j) {
+    const rvsp_csr_matrix_t& Bj = B[bs.b_pair[j]];
+    const long long rows = Bj.rows + (j == nb - 1 ? 1 : 0);
+    t.add(Bj.row_ptr, (int*)bs.b_rp.p + bs.b_row_off[j], rows, (int)bs.b_nnz_off[j], 1);
+    t.add(Bj.col_idx, (int*)bs.b_ci.p + bs.b_nnz_off[j], Bj.nnz, 0, 0);
+  }
+}
+
+void build_value_table(BatchState& bs, const rvsp_csr_matrix_t* A, const rvsp_csr_matrix_t* B) {
+  SegTable& t = bs.t_vals;
+  t.clear();
+  for (int p = 0; p < bs.count; ++p)
+    t.add((const int*)A[p].values, (int*)bs.a_val.p + bs.a_nnz_off[p], A[p].nnz, 0, 0);
+  for (int j = 0; j < (int)bs.b_pair.size(); ++j) {
+    const rvsp_csr_matrix_t& Bj = B[bs.b_pair[j]];
+    t.add((const int*)Bj.values, (int*)bs.b_val.p + bs.b_nnz_off[j], Bj.nnz, 0, 0);
+  }
+}
+
+std::vector<const void*> struct_key(int n, const rvsp_csr_matrix_t* A,
+                                    const rvsp_csr_matrix_t* B) {
+  std::vector<const void*> k;
+  k.reserve((size_t)n * 4);
+  for (int p = 0; p < n; ++p) {
+    k.push_back(A[p].row_ptr);
+    k.push_back(A[p].col_idx);
+    k.push_back(B[p].row_ptr);
+    k.push_back(B[p].col_idx);
+  }
+  return k;
+}
+
+std::vector<const void*> value_key(int n, const rvsp_csr_matrix_t* A,
+                                   const rvsp_csr_matrix_t* B) {
+  std::vector<const void*> k;
+  k.reserve((size_t)n * 2);
+  for (int p = 0; p < n; ++p) {
+    k.push_back(A[p].values);
+    k.push_back(B[p].values);
+  }
+  return k;
+}
+
+std::vector<const void*> output_key(int n, const rvsp_csr_matrix_t* C) {
+  std::vector<const void*> k;
+  k.reserve((size_t)n * 3);
+  for (int p = 0; p < n; ++p) {
+    k.push_back(C[p].row_ptr);
+    k.push_back(C[p].col_idx);
+    k.push_back(C[p].values);
+  }
+  return k;
+}
+
+rvsp_status_t check_pairs(int count, const rvsp_csr_matrix_t* A, const rvsp_csr_matrix_t* B) {
+  for (int p = 0; p < count; ++p) {
+    const rvsp_csr_matrix_t* ops[] = {&A[p], &B[p]};
+    for (const rvsp_csr_matrix_t* M : ops) {
+      if (M->row_ptr == NULL || M->col_idx == NULL || M->values == NULL)
+        return RVSP_ERROR_NULL_POINTER;
+      if (M->dtype != RVSP_DTYPE_FP32) return RVSP_ERROR_UNSUPPORTED_DTYPE;
+      if (M->rows <= 0 || M->cols <= 0 || M->nnz < 0) return RVSP_ERROR_INVALID_ARGUMENT;
+    }
+    if (A[p].cols != B[p].rows) return RVSP_ERROR_INVALID_ARGUMENT;
+  }
+  return RVSP_SUCCESS;
+}
+
+// Output table of the current C arrays: row pointers of every pair always;
+// column indices and values of the groups that do not write in place.
+void build_output_tables(BatchState& bs, const rvsp_csr_matrix_t* C) {
+  bs.t_out_full.clear();
+  bs.t_out_vals.clear();
+  for (BatchGroup& g : bs.groups) {
+    for (int p = g.p0; p < g.p1; ++p) {
+      const long long lr = bs.a_row_off[p] - g.r0;
+      bs.t_out_full.add((const int*)g.c_rp.p + lr, C[p].row_ptr, (long long)C[p].rows + 1, 0, 1);
+      if (g.in_place) continue;
+      const long long o = g.off[p - g.p0];
+      const long long n = g.off[p - g.p0 + 1] - o;
+      bs.t_out_full.add((const int*)g.c_ci.p + o, C[p].col_idx, n, 0, 0);
+      bs.t_out_full.add((const int*)g.c_val.p + o, (int*)C[p].values, n, 0, 0);
+      bs.t_out_vals.add((const int*)g.c_val.p + o, (int*)C[p].values, n, 0, 0);
+    }
+  }
+}
+
+// The compute body: values into the stack, every group's compute, results
+// out. Launches only, so it can be captured.
+rvsp_status_t batch_body(BatchState& bs, const rvsp_csr_matrix_t* C, bool full_out,
+                         hipStream_t s) {
+  RVSP_HIP_TRY(bs.t_vals.launch(s));
+  const rvsp_csr_matrix_t Bst = view((int)bs.b_row_off.back(), bs.b_cols, bs.b_nnz_off.back(),
+                                     bs.b_rp.p, bs.b_ci.p, bs.b_val.p);
+  for (BatchGroup& g : bs.groups) {
+    const rvsp_csr_matrix_t Ast =
+        view((int)(g.r1 - g.r0), (int)bs.b_row_off.back(),
+             bs.a_nnz_off[g.p1] - bs.a_nnz_off[g.p0], (const int*)bs.a_rp.p + g.r0, bs.a_ci.p,
+             bs.a_val.p);
+    const void* ci = g.in_place ? (const void*)C[g.p0].col_idx : g.c_ci.p;
+    const void* val = g.in_place ? (const void*)C[g.p0].values : g.c_val.p;
+    rvsp_csr_matrix_t Cg = view(Ast.rows, bs.b_cols, g.c_nnz, g.c_rp.p, ci, val);
+    char token = 0;
+    RVSP_TRY(rvsp_spgemm_compute(g.inner, &Ast, &Bst, &Cg, &token));
+  }
+  const SegTable& out = full_out ? bs.t_out_full : bs.t_out_vals;
+  if (!out.chunks.empty()) RVSP_HIP_TRY(out.launch(s));
+  return RVSP_SUCCESS;
+}
+
+}  // namespace
+
+}  // namespace rvsp
+
+using namespace rvsp;
+
+extern "C" {
+
+rvsp_status_t rvsp_spgemm_batch_work_estimation(rvsp_spgemm_descr_t descr, int32_t count,
+                                                const rvsp_csr_matrix_t* A,
+                                                const rvsp_csr_matrix_t* B, int32_t* c_nnz) {
+  if (descr == NULL || A == NULL || B == NULL || c_nnz == NULL) return RVSP_ERROR_NULL_POINTER;
+  if (count < 1) return RVSP_ERROR_INVALID_ARGUMENT;
+  if (descr->memory != RVSP_MEMORY_DEVICE) return RVSP_ERROR_INVALID_ARGUMENT;
+  RVSP_TRY(check_pairs(count, A, B));
+  rvsp_spgemm_descr* d = descr;
+  destroy_batch(d);
+  d->batch = new (std::nothrow) BatchState();
+  if (!d->batch) return RVSP_ERROR_ALLOCATION_FAILED;
+  BatchState& bs = *d->batch;
+  RVSP_TRY(device_ready());
+  hipStream_t s;
+  RVSP_TRY(descr_stream(d, &s));
+
+  bs.count = count;
+  bs.a.assign(A, A + count);
+  bs.b.assign(B, B + count);
+  // Stack offsets; B operands that are the same arrays are stacked once.
+  bs.b_of.resize(count);
+  {
+    using BKey = std::tuple<const void*, const void*, const void*, int, int, int>;
+    std::map<BKey, int> seen;
+    for (int p = 0; p < count; ++p) {
+      const BKey k(B[p].row_ptr, B[p].col_idx, B[p].values, B[p].rows, B[p].cols, B[p].nnz);
+      auto it = seen.find(k);
+      if (it == seen.end()) {
+        it = seen.emplace(k, (int)bs.b_pair.size()).first;
+        bs.b_pair.push_back(p);
+      }
+      bs.b_of[p] = it->second;
+    }
+  }
+  const int nb = (int)bs.b_pair.size();
+  bs.a_row_off.assign(count + 1, 0);
+  bs.a_nnz_off.assign(count + 1, 0);
+  bs.b_row_off.assign(nb + 1, 0);
+  bs.b_nnz_off.assign(nb + 1, 0);
+  for (int p = 0; p < count; ++p) {
+    bs.a_row_off[p + 1] = bs.a_row_off[p] + A[p].rows;
+    bs.a_nnz_off[p + 1] = bs.a_nnz_off[p] + A[p].nnz;
+  }
+  bs.b_cols = 1;
+  for (int j = 0; j < nb; ++j) {
+    const rvsp_csr_matrix_t& Bj = B[bs.b_pair[j]];
+    bs.b_row_off[j + 1] = bs.b_row_off[j] + Bj.rows;
+    bs.b_nnz_off[j + 1] = bs.b_nnz_off[j] + Bj.nnz;
+    bs.b_cols = std::max(bs.b_cols, Bj.cols);
+  }
+  // The stacks are int32-indexed like every operand.
+  if (bs.a_row_off[count] >= INT_MAX || bs.a_nnz_off[count] > INT_MAX ||
+      bs.b_row_off[nb] >= INT_MAX || bs.b_nnz_off[nb] > INT_MAX)
+    return RVSP_ERROR_INVALID_ARGUMENT;
+  RVSP_TRY(ensure(bs.a_rp, ((size_t)bs.a_row_off[count] + 1) * 4));
+  RVSP_TRY(ensure(bs.a_ci, (size_t)bs.a_nnz_off[count] * 4));
+  RVSP_TRY(ensure(bs.a_val, (size_t)bs.a_nnz_off[count] * 4));
+  RVSP_TRY(ensure(bs.b_rp, ((size_t)bs.b_row_off[nb] + 1) * 4));
+  RVSP_TRY(ensure(bs.b_ci, (size_t)bs.b_nnz_off[nb] * 4));
+  RVSP_TRY(ensure(bs.b_val, (size_t)bs.b_nnz_off[nb] * 4));
+  build_struct_table(bs, A, B);
+  RVSP_TRY(bs.t_struct.upload(s));
+  RVSP_HIP_TRY(bs.t_struct.launch(s));
+  bs.struct_key = struct_key(count, A, B);
+
+  // Products per pair, then groups of consecutive pairs under the limit.
+  std::vector<OpsChunk> oc;
+  for (int p = 0; p < count; ++p)
+    for (long long b = bs.a_nnz_off[p]; b < bs.a_nnz_off[p + 1]; b += 16 * kSegChunk) {
+      OpsChunk c;
+      c.pair = p;
+      c.pad = 0;
+      c.begin = b;
+      c.end = std::min(bs.a_nnz_off[p + 1], b + 16 * kSegChunk);
+      oc.push_back(c);
+    }
+  std::vector<unsigned long long> ops(count, 0);
+  RVSP_TRY(ensure(bs.ops, (size_t)count * 8));
+  RVSP_HIP_TRY(hipMemsetAsync(bs.ops.p, 0, (size_t)count * 8, s));
+  DevBuf d_oc;
+  if (!oc.empty()) {
+    RVSP_TRY(ensure(d_oc, oc.size() * sizeof(OpsChunk)));
+    hipError_t e = hipMemcpyAsync(d_oc.p, oc.data(), oc.size() * sizeof(OpsChunk),
+                                  hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = launch_pair_ops((const OpsChunk*)d_oc.p, (int)oc.size(), (const int*)bs.a_ci.p,
+                          (const int*)bs.b_rp.p, (unsigned long long*)bs.ops.p, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(ops.data(), bs.ops.p, (size_t)count * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    release(d_oc);
+    RVSP_HIP_TRY(e);
+  }
+  const long long limit = d->batch_limit > 0 ? d->batch_limit : INT_MAX;
+  {
+    BatchGroup g;
+    long long acc = 0;
+    for (int p = 0; p < count; ++p) {
+      if (p > g.p0 && acc + (long long)ops[p] > limit) {
+        g.p1 = p;
+        bs.groups.push_back(g);
+        g = BatchGroup();
+        g.p0 = p;
+        acc = 0;
+      }
+      acc += (long long)ops[p];
+    }
+    g.p1 = count;
+    bs.groups.push_back(g);
+  }
+
+  // One inner descriptor per group over its rows of the stacked A.
+  bs.c_nnz.assign(count, 0);
+  const rvsp_csr_matrix_t Bst = view((int)bs.b_row_off[nb], bs.b_cols, bs.b_nnz_off[nb],
+                                     bs.b_rp.p, bs.b_ci.p, bs.b_val.p);
+  for (BatchGroup& g : bs.groups) {
+    g.r0 = bs.a_row_off[g.p0];
+    g.r1 = bs.a_row_off[g.p1];
+    RVSP_TRY(rvsp_spgemm_descr_create(&g.inner));
+    RVSP_TRY(rvsp_spgemm_set_algo(g.inner, d->algo));
+    RVSP_TRY(rvsp_spgemm_set_memory(g.inner, RVSP_MEMORY_DEVICE));
+    RVSP_TRY(rvsp_spgemm_set_stream(g.inner, (void*)s));
+    RVSP_TRY(rvsp_spgemm_set_async(g.inner, 1));
+    if (!d->choices.empty())
+      RVSP_TRY(rvsp_spgemm_set_bin_choices(g.inner, d->choices.data(),
+                                           (int32_t)d->choices.size()));
+    const rvsp_csr_matrix_t Ast =
+        view((int)(g.r1 - g.r0), (int)bs.b_row_off[nb], bs.a_nnz_off[g.p1] - bs.a_nnz_off[g.p0],
+             (const int*)bs.a_rp.p + g.r0, bs.a_ci.p, bs.a_val.p);
+    size_t ws = 0;
+    RVSP_TRY(rvsp_spgemm_work_estimation(g.inner, &Ast, &Bst, &ws, &g.c_nnz));
+    // Pair boundaries inside the group's row pointers.
+    const int np = g.p1 - g.p0;
+    std::vector<long long> idx(np + 1);
+    for (int i = 0; i <= np; ++i) idx[i] = bs.a_row_off[g.p0 + i] - g.r0;
+    std::vector<int> at(np + 1);
+    RVSP_TRY(ensure(bs.gather_idx, (size_t)(np + 1) * 8));
+    RVSP_TRY(ensure(bs.gather_out, (size_t)(np + 1) * 4));
+    RVSP_HIP_TRY(hipMemcpyAsync(bs.gather_idx.p, idx.data(), (size_t)(np + 1) * 8,
+                                hipMemcpyHostToDevice, s));
+    RVSP_HIP_TRY(launch_gather_i32((const int*)g.inner->c_rp.p, (const long long*)bs.gather_idx.p,
+                                   np + 1, (int*)bs.gather_out.p, s));
+    RVSP_HIP_TRY(hipMemcpyAsync(at.data(), bs.gather_out.p, (size_t)(np + 1) * 4,
+                                hipMemcpyDeviceToHost, s));
+    RVSP_HIP_TRY(hipStreamSynchronize(s));
+    g.off.assign(np + 1, 0);
+    for (int i = 0; i <= np; ++i) g.off[i] = at[i];
+    for (int i = 0; i < np; ++i) bs.c_nnz[g.p0 + i] = at[i + 1] - at[i];
+    RVSP_TRY(ensure(g.c_rp, (size_t)(g.r1 - g.r0 + 1) * 4));
+  }
+  for (int p = 0; p < count; ++p) c_nnz[p] = bs.c_nnz[p];
+  bs.estimated = true;
+  return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_spgemm_batch_compute(rvsp_spgemm_descr_t descr, int32_t count,
+                                        const rvsp_csr_matrix_t* A, const rvsp_csr_matrix_t* B,
+                                        rvsp_csr_matrix_t* C) {
+  if (descr == NULL || A == NULL || B == NULL || C == NULL) return RVSP_ERROR_NULL_POINTER;
+  rvsp_spgemm_descr* d = descr;
+  if (!d->batch || !d->batch->estimated) return RVSP_ERROR_INVALID_ARGUMENT;
+  BatchState& bs = *d->batch;
+  if (count != bs.count) return RVSP_ERROR_INVALID_ARGUMENT;
+  RVSP_TRY(check_pairs(count, A, B));
+  for (int p = 0; p < count; ++p) {
+    const rvsp_csr_matrix_t &a = bs.a[p], &b = bs.b[p];
+    if (A[p].rows != a.rows || A[p].cols != a.cols || A[p].nnz != a.nnz ||
+        B[p].rows != b.rows || B[p].cols != b.cols || B[p].nnz != b.nnz)
+      return RVSP_ERROR_INVALID_ARGUMENT;
+  }
+  for (int p = 0; p < count; ++p) {
+    if (C[p].row_ptr == NULL || C[p].col_idx == NULL || C[p].values == NULL)
+      return RVSP_ERROR_NULL_POINTER;
+    if (C[p].dtype != RVSP_DTYPE_FP32) return RVSP_ERROR_UNSUPPORTED_DTYPE;
+    if (C[p].rows != A[p].rows || C[p].cols != B[p].cols) return RVSP_ERROR_INVALID_ARGUMENT;
+    if (C[p].nnz < bs.c_nnz[p]) return RVSP_ERROR_INVALID_ARGUMENT;
+  }
+  // Stacked B rows are shared by the pairs that gave the same B arrays at
+  // estimation; a compute must keep that sharing.
+  for (int p = 0; p < count; ++p)
+    if (!same_matrix(B[p], B[bs.b_pair[bs.b_of[p]]])) return RVSP_ERROR_INVALID_ARGUMENT;
+
+  hipStream_t s;
+  RVSP_TRY(descr_stream(d, &s));
+  // Structure arrays moved: stack them again (same shapes, same offsets).
+  std::vector<const void*> sk = struct_key(count, A, B);
+  if (sk != bs.struct_key) {
+    bs.drop_graph();
+    build_struct_table(bs, A, B);
+    RVSP_TRY(bs.t_struct.upload(s));
+    RVSP_HIP_TRY(bs.t_struct.launch(s));
+    bs.struct_key = sk;
+    for (BatchGroup& g : bs.groups) rvsp_spgemm_invalidate_structure(g.inner);
+  }
+  std::vector<const void*> vk = value_key(count, A, B);
+  if (vk != bs.vals_key) {
+    bs.drop_graph();
+    build_value_table(bs, A, B);
+    RVSP_TRY(bs.t_vals.upload(s));
+    bs.vals_key = vk;
+  }
+  std::vector<const void*> ok = output_key(count, C);
+  if (ok != bs.out_key) {
+    bs.drop_graph();
+    for (BatchGroup& g : bs.groups) {
+      bool in_place = true;
+      for (int p = g.p0 + 1; p < g.p1 && in_place; ++p) {
+        const long long o = g.off[p - g.p0];
+        in_place = C[p].col_idx == C[g.p0].col_idx + o &&
+                   (const float*)C[p].values == (const float*)C[g.p0].values + o;
+      }
+      g.in_place = in_place;
+      if (!in_place) {
+        RVSP_TRY(ensure(g.c_ci, (size_t)std::max(1, g.c_nnz) * 4));
+        RVSP_TRY(ensure(g.c_val, (size_t)std::max(1, g.c_nnz) * 4));
+      }
+    }
+    build_output_tables(bs, C);
+    RVSP_TRY(bs.t_out_full.upload(s));
+    RVSP_TRY(bs.t_out_vals.upload(s));
+    bs.out_key = ok;
+    bs.out_filled = false;
+  }
+
+  std::vector<const void*> key = sk;
+  key.insert(key.end(), vk.begin(), vk.end());
+  key.insert(key.end(), ok.begin(), ok.end());
+  const bool repeat = bs.out_filled && key == bs.last_key;
+  bs.last_key = key;
+  if (repeat && d->batch_graph && !bs.graph_failed) {
+    if (!bs.exec) {
+      // Capture one numeric-only compute; later ones replay it.
+      RVSP_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const rvsp_status_t st = batch_body(bs, C, false, s);
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(s, &g);
+      hipGraphExec_t ex = nullptr;
+      hipError_t ei = hipErrorUnknown;
+      if (st == RVSP_SUCCESS && ec == hipSuccess && g)
+        ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      if (ei == hipSuccess) {
+        bs.graph = g;
+        bs.exec = ex;
+      } else {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        bs.graph_failed = true;
+      }
+    }
+    if (bs.exec) {
+      RVSP_HIP_TRY(hipGraphLaunch(bs.exec, s));
+      ++bs.graph_launches;
+    } else {
+      RVSP_TRY(batch_body(bs, C, false, s));
+    }
+  } else {
+    RVSP_TRY(batch_body(bs, C, !bs.out_filled, s));
+    bs.out_filled = true;
+  }
+  if (!d->async) RVSP_HIP_TRY(hipStreamSynchronize(s));
+  for (int p = 0; p < count; ++p) {
+    C[p].nnz = bs.c_nnz[p];
+    C[p].format = RVSP_FORMAT_CSR;
+  }
+  return RVSP_SUCCESS;
+}
+
+int32_t rvsp_spgemm_batch_groups(rvsp_spgemm_descr_t descr) {
+  if (descr == NULL || !descr->batch || !descr->batch->estimated) return -1;
+  return (int32_t)descr->batch->groups.size();
+}
+
+int32_t rvsp_spgemm_batch_graph_launches(rvsp_spgemm_descr_t descr) {
+  if (descr == NULL || !descr->batch) return 0;
+  return descr->batch->graph_launches;
+}
+
+rvsp_status_t rvsp_spgemm_batch_set_graph(rvsp_spgemm_descr_t descr, int on) {
+  if (descr == NULL) return RVSP_ERROR_NULL_POINTER;
+  descr->batch_graph = on != 0;
+  if (descr->batch) descr->batch->drop_graph();
+  return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_spgemm_batch_set_group_limit(rvsp_spgemm_descr_t descr,
+                                                int64_t max_products) {
+  if (descr == NULL) return RVSP_ERROR_NULL_POINTER;
+  if (max_products < 0 || max_products > INT_MAX) return RVSP_ERROR_INVALID_ARGUMENT;
+  if (descr->batch && descr->batch->estimated) return RVSP_ERROR_INVALID_ARGUMENT;
+  descr->batch_limit = max_products;
+  return RVSP_SUCCESS;
+}
+
+}  // extern "C"

@@ -482,6 +482,7 @@ void rvsp_spgemm_descr_destroy(rvsp_spgemm_descr_t descr) {
                     &d->map, &d->poff, &d->map_weights};
   for (DevBuf* b : bufs) release(*b);
   destroy_graph(d);
+  destroy_batch(d);
   if (d->own_stream) (void)hipStreamDestroy(d->own_stream);
   free(d->c_row_ptr);
   delete d;
@@ -491,7 +492,7 @@ void rvsp_spgemm_descr_destroy(rvsp_spgemm_descr_t descr) {
 rvsp_status_t rvsp_spgemm_set_algo(rvsp_spgemm_descr_t descr,
                                    rvsp_spgemm_algo_t algo) {
   if (descr == NULL) return RVSP_ERROR_NULL_POINTER;
-  if (descr->estimated) return RVSP_ERROR_INVALID_ARGUMENT;
+  if (descr->estimated || descr->batch) return RVSP_ERROR_INVALID_ARGUMENT;
   if (!algo_available(algo)) return RVSP_ERROR_UNSUPPORTED_BACKEND;
   descr->algo = algo;
   return RVSP_SUCCESS;
@@ -503,6 +504,7 @@ void rvsp_spgemm_invalidate_structure(rvsp_spgemm_descr_t descr) {
     descr->filled_into = NULL;
     descr->dev_cols_valid = false;
     descr->map_valid = false;
+    invalidate_batch(descr);
   }
 }
 

@@ -14,6 +14,8 @@
 
 namespace rvsp {
 
+struct BatchState;
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -82,6 +84,12 @@ struct rvsp_spgemm_descr {
   unsigned map_bins = 0;    // numeric bins whose rows have map entries
   bool map_valid = false;   // entries built for the current structure
 
+  // Batched products (batch.cpp): stacked operands and one inner
+  // descriptor per group of pairs; settings apply at the next estimation.
+  rvsp::BatchState* batch = nullptr;
+  bool batch_graph = true;
+  long long batch_limit = 0;  // products per group; 0 = INT32_MAX
+
   // Captured numeric-only graph.
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -105,5 +113,7 @@ rvsp_status_t ensure_map_built(rvsp_spgemm_descr* d, const DevCsr& A, const DevC
                                const int* c_ci, hipStream_t s);
 
 void destroy_graph(rvsp_spgemm_descr* d);
+void destroy_batch(rvsp_spgemm_descr* d);
+void invalidate_batch(rvsp_spgemm_descr* d);
 
 }  // namespace rvsp

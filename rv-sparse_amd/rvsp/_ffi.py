@@ -125,6 +125,12 @@ PROTOTYPES = {
     "rvsp_tuning_table_load": (I32, [C.c_char_p]),
     "rvsp_tuning_table_clear": (None, []),
     "rvsp_tuning_table_size": (I32, []),
+    "rvsp_spgemm_batch_work_estimation": (C.c_int, [P, I32, PM, PM, I32P]),
+    "rvsp_spgemm_batch_compute": (C.c_int, [P, I32, PM, PM, PM]),
+    "rvsp_spgemm_batch_groups": (I32, [P]),
+    "rvsp_spgemm_batch_graph_launches": (I32, [P]),
+    "rvsp_spgemm_batch_set_graph": (C.c_int, [P, C.c_int]),
+    "rvsp_spgemm_batch_set_group_limit": (C.c_int, [P, I64]),
     "rvsp_hip_malloc": (C.c_int, [C.POINTER(P), SZ]),
     "rvsp_hip_free": (C.c_int, [P]),
     "rvsp_hip_memcpy": (C.c_int, [P, P, SZ, C.c_int]),

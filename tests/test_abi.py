@@ -192,3 +192,24 @@ def test_no_device_fails_loudly():
     with pytest.raises(rvsp.RvspError) as e:
         rvsp.spgemm(A, A)
     assert e.value.status == F.ERROR_DEVICE
+
+
+def test_batch_entry_points_check_arguments_without_a_device():
+    h = C.c_void_p()
+    F.check(L.rvsp_spgemm_descr_create(C.byref(h)))
+    a = (F.CsrMatrix * 1)()
+    out = (C.c_int32 * 1)()
+    assert L.rvsp_spgemm_batch_work_estimation(None, 1, a, a, out) == F.ERROR_NULL_POINTER
+    assert L.rvsp_spgemm_batch_work_estimation(h, 1, a, a, None) == F.ERROR_NULL_POINTER
+    assert L.rvsp_spgemm_batch_work_estimation(h, 0, a, a, out) == F.ERROR_INVALID_ARGUMENT
+    # Host memory (the default) has no batched path.
+    assert L.rvsp_spgemm_batch_work_estimation(h, 1, a, a, out) == F.ERROR_INVALID_ARGUMENT
+    assert L.rvsp_spgemm_batch_compute(h, 1, a, a, a) == F.ERROR_INVALID_ARGUMENT
+    assert L.rvsp_spgemm_batch_groups(h) == -1
+    assert L.rvsp_spgemm_batch_graph_launches(h) == 0
+    assert L.rvsp_spgemm_batch_set_group_limit(h, -1) == F.ERROR_INVALID_ARGUMENT
+    assert L.rvsp_spgemm_batch_set_group_limit(h, 1 << 40) == F.ERROR_INVALID_ARGUMENT
+    assert L.rvsp_spgemm_batch_set_group_limit(h, 1000) == F.SUCCESS
+    assert L.rvsp_spgemm_batch_set_graph(h, 0) == F.SUCCESS
+    assert L.rvsp_spgemm_batch_set_graph(None, 0) == F.ERROR_NULL_POINTER
+    L.rvsp_spgemm_descr_destroy(h)

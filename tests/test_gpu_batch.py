@@ -1,0 +1,163 @@
+"""Batched products (rvsp_spgemm_batch_*): every pair bit-identical to the
+oracle, across groups, output layouts, shared B operands, repeated
+numeric-only computes replayed from a HIP graph, and the argument checks."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import rvsp
+from oracle import oracle
+from rvsp import _ffi as F
+from rvsp import gen
+from rvsp.api import Csr
+from rvsp.batch import Batch, BatchOutput, spgemm_batch, upload_pairs
+
+pytestmark = pytest.mark.gpu
+L = F.lib()
+
+
+def small_pairs(count=24, seed=0, n_min=16, n_max=2048):
+    return gen.batch_pairs(count, n_min=n_min, n_max=n_max, seed=seed)
+
+
+def assert_pairs_match(pairs, got):
+    for p, ((A, B), g) in enumerate(zip(pairs, got)):
+        want = oracle.spgemm(A, B)
+        for name, x, w in zip(("row_ptr", "col_idx", "values"),
+                              (g.row_ptr, g.col_idx, g.values), want):
+            assert x.tobytes() == w.tobytes(), f"pair {p} {name}"
+
+
+@pytest.mark.parametrize("contiguous", [True, False])
+def test_batch_matches_oracle_per_pair(gpu, contiguous):
+    pairs = small_pairs()
+    got, groups = spgemm_batch(pairs, contiguous=contiguous)
+    assert groups == 1
+    assert_pairs_match(pairs, got)
+
+
+@pytest.mark.parametrize("contiguous", [True, False])
+def test_batch_groups_under_a_product_limit(gpu, contiguous):
+    pairs = small_pairs(count=20, seed=1)
+    ops = [int(np.diff(B.row_ptr)[A.col_idx].sum()) for A, B in pairs]
+    limit = max(max(ops), sum(ops) // 5)
+    got, groups = spgemm_batch(pairs, contiguous=contiguous, group_limit=limit)
+    assert groups >= 4
+    assert_pairs_match(pairs, got)
+
+
+def test_batch_of_row_blocks_shares_b(gpu):
+    """C = A*A as row blocks of A, all against the same B arrays: B is
+    stacked once and every block equals the matching rows of the product."""
+    A = gen.rmat(12, seed=4)
+    cuts = [0, 700, 701, 2500, A.rows]
+    blocks = []
+    for r0, r1 in zip(cuts[:-1], cuts[1:]):
+        rp = A.row_ptr[r0:r1 + 1] - A.row_ptr[r0]
+        s, e = A.row_ptr[r0], A.row_ptr[r1]
+        blocks.append(Csr(r1 - r0, A.cols, rp, A.col_idx[s:e], A.values[s:e]))
+    pairs = [(blk, A) for blk in blocks]
+    got, _ = spgemm_batch(pairs)
+    full = oracle.spgemm(A, A)
+    rp, ci, v = full
+    for (r0, r1), g in zip(zip(cuts[:-1], cuts[1:]), got):
+        s, e = rp[r0], rp[r1]
+        assert np.array_equal(g.row_ptr, rp[r0:r1 + 1] - s)
+        assert g.col_idx.tobytes() == ci[s:e].tobytes()
+        assert g.values.tobytes() == v[s:e].tobytes()
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_repeated_numeric_computes_with_new_values(gpu, graph):
+    pairs = small_pairs(count=12, seed=2)
+    As, Bs, owned = upload_pairs(pairs)
+    bt = Batch(graph=graph)
+    out = None
+    try:
+        nnz = bt.estimate(As, Bs)
+        out = BatchOutput([a.rows for a, _ in pairs], [b.cols for _, b in pairs], nnz)
+        rng = np.random.default_rng(7)
+        for it in range(4):
+            if it:
+                # New values in the same device arrays: numeric-only computes.
+                new = []
+                for (A, B), dA, dB in zip(pairs, As, Bs):
+                    a2 = A.with_values(rng.uniform(-1, 1, A.nnz).astype(np.float32))
+                    b2 = B.with_values(rng.uniform(-1, 1, B.nnz).astype(np.float32))
+                    dA.set_values(a2.values)
+                    dB.set_values(b2.values)
+                    new.append((a2, b2))
+                pairs = new
+            F.check(bt.compute(As, Bs, out.structs), "batch_compute")
+            assert_pairs_match(pairs, [out.download(p) for p in range(len(pairs))])
+        # The first compute fills, the second is captured and launched as a
+        # graph, later ones replay it.
+        assert bt.graph_launches() == (3 if graph else 0)
+    finally:
+        if out is not None:
+            out.free()
+        bt.close()
+        for d in owned:
+            d.free()
+
+
+def test_batch_edge_pairs(gpu):
+    """Pairs with empty rows, an all-empty product and 1x1 operands."""
+    empty = Csr(5, 7, np.zeros(6, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32))
+    B = Csr.from_dense(np.arange(63, dtype=np.float32).reshape(7, 9) % 3)
+    A = Csr.from_dense(np.array([[0, 1, 0, 0, 0, 0, 2], [0] * 7, [3, 0, 0, 0, 0, 0, 0]],
+                                np.float32))
+    one = Csr.from_dense(np.array([[2.0]], np.float32))
+    pairs = [(A, B), (empty, B), (one, one), (A, B)]
+    got, _ = spgemm_batch(pairs)
+    assert_pairs_match(pairs, got)
+    assert got[1].nnz == 0
+
+
+def test_batch_argument_checks(gpu):
+    pairs = small_pairs(count=3, seed=3)
+    As, Bs, owned = upload_pairs(pairs)
+    try:
+        a = Batch._structs(As)
+        b = Batch._structs(Bs)
+        out = (C.c_int32 * 3)()
+        h = C.c_void_p()
+        F.check(L.rvsp_spgemm_descr_create(C.byref(h)))
+        # Host-memory descriptors have no batched path.
+        assert L.rvsp_spgemm_batch_work_estimation(h, 3, a, b, out) == F.ERROR_INVALID_ARGUMENT
+        F.check(L.rvsp_spgemm_set_memory(h, F.MEMORY_DEVICE))
+        assert L.rvsp_spgemm_batch_compute(h, 3, a, b, (F.CsrMatrix * 3)()) == \
+            F.ERROR_INVALID_ARGUMENT  # before estimation
+        assert L.rvsp_spgemm_batch_work_estimation(h, 0, a, b, out) == F.ERROR_INVALID_ARGUMENT
+        assert L.rvsp_spgemm_batch_work_estimation(h, 3, None, b, out) == F.ERROR_NULL_POINTER
+        bad = Batch._structs(Bs)
+        bad[1].rows += 1  # A[1].cols != B[1].rows
+        assert L.rvsp_spgemm_batch_work_estimation(h, 3, a, bad, out) == F.ERROR_INVALID_ARGUMENT
+        F.check(L.rvsp_spgemm_batch_work_estimation(h, 3, a, b, out))
+        assert L.rvsp_spgemm_batch_groups(h) == 1
+        nnz = np.array(out[:])
+        o = BatchOutput([x.rows for x, _ in pairs], [y.cols for _, y in pairs], nnz)
+        try:
+            assert L.rvsp_spgemm_batch_compute(h, 2, a, b, o.structs) == F.ERROR_INVALID_ARGUMENT
+            small = (F.CsrMatrix * 3)(*o.structs)
+            small[2].nnz = int(nnz[2]) - 1
+            if nnz[2] > 0:
+                assert L.rvsp_spgemm_batch_compute(h, 3, a, b, small) == F.ERROR_INVALID_ARGUMENT
+            F.check(L.rvsp_spgemm_batch_compute(h, 3, a, b, o.structs))
+            assert_pairs_match(pairs, [o.download(p) for p in range(3)])
+            # set_algo after a batch estimation is refused, as after one.
+            assert L.rvsp_spgemm_set_algo(h, F.ALGO_HIP_FIXED) == F.ERROR_INVALID_ARGUMENT
+        finally:
+            o.free()
+        L.rvsp_spgemm_descr_destroy(h)
+    finally:
+        for d in owned:
+            d.free()
+
+
+def test_batch_learned_and_tuned_algos_keep_bits(gpu):
+    pairs = small_pairs(count=10, seed=5)
+    for algo in (F.ALGO_HIP_TUNED, F.ALGO_HIP_LEARNED):
+        got, _ = spgemm_batch(pairs, algo=algo)
+        assert_pairs_match(pairs, got)
