@@ -30,6 +30,8 @@ import statistics
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "rv-sparse_amd"))
 sys.path.insert(0, ROOT)
@@ -140,42 +142,412 @@ def run_plan(hip, A, dA, val_bufs, algo, steps, warmup, autotune_reps):
             "t_tune": t_tune, "plan": plan, "nnz_c": nnz_c, "C": dC}
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--grid", type=int, default=1024)
-    ap.add_argument("--autotune-reps", type=int, default=5)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-verify", action="store_true")
-    ap.add_argument("--only", choices=["fixed", "tuned"], default=None,
-                    help="time one plan only (profiling runs)")
-    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
-                    help="only run device-to-device copies of this many MiB (a known "
-                         "byte count for scaling the HBM traffic counters), then exit")
-    args = ap.parse_args()
-    if args.calibrate_copy_mib:
-        import rvsp
-        n = args.calibrate_copy_mib << 20
-        src, dst = rvsp.device_alloc(n), rvsp.device_alloc(n)
-        for _ in range(3):
-            rvsp._ffi.check(rvsp.lib().rvsp_hip_memcpy(C.c_void_p(dst), C.c_void_p(src), n, 3))
-        rvsp.synchronize()
-        print(json.dumps({"calibration_copy_bytes": n, "copies": 3}))
-        return
-    if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has one MI355X, and "
-              "the SpGEMM path has no multi-GPU mode (DESIGN.md describes how it would "
-              "shard by output rows).", file=sys.stderr)
-        sys.exit(2)
+# ---------------------------------------------------------------------------
+# Secondary workloads: reported under "configs" next to the headline.
 
+def cpu_oneshot(pairs, budget_s):
+    """One-shot CPU products (reference raw scalar kernel when built, else
+    the oracle restatement) over a prefix of `pairs` that fits the budget.
+    Returns (seconds, products, n_pairs, kind)."""
+    from oracle import oracle
+    kind = "reference" if oracle.ref_available() else "port"
+    spent, products, n = 0.0, 0, 0
+    for A, B in pairs:
+        t0 = time.perf_counter()
+        if kind == "reference":
+            oracle.ref_spgemm(A, B)
+        else:
+            oracle.spgemm(A, B)
+        spent += time.perf_counter() - t0
+        products += int(np.diff(B.row_ptr)[A.col_idx].sum())
+        n += 1
+        if spent > budget_s:
+            break
+    return spent, products, n, kind
+
+
+def word_checksum(t, offset=0, chunk=1 << 27):
+    """Order-sensitive checksum (mod 2^64) of a 32-bit device tensor's words,
+    as if the tensor started at word `offset` of a longer array (so per-block
+    sums add up to the sum over the concatenation). Chunked to bound the
+    temporaries."""
+    import torch
+    w32 = t.reshape(-1).view(torch.int32)
+    total = 0
+    for s in range(0, w32.numel(), chunk):
+        w = w32[s:s + chunk].long()
+        idx = (torch.arange(w.numel(), device=w.device, dtype=torch.long) + offset + s) * 2 + 1
+        total += int((w * idx).sum())
+    return total % (1 << 64)
+
+
+def bench_batched(hip, args, F):
+    """2048 independent pairs of uneven shapes: the plain path (one
+    descriptor per pair, FIXED) against one batched descriptor (one set of
+    launches per phase; numeric-only repeats replayed from a HIP graph)."""
+    import torch
     import rvsp
-    from rvsp import _ffi as F
-    from rvsp import gen
+    from oracle import oracle
+    from rvsp import devgen
+    from rvsp.batch import Batch
+    L = F.lib()
+    t0 = time.perf_counter()
+    pairs, keep = devgen.lognormal_batch(count=args.pairs, seed=0, device="cuda")
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t0
+    products = 0
+    for A, B in pairs:
+        products += int((B.rp[1:] - B.rp[:-1]).long()[A.ci.long()].sum())
+    print(f"# batched: {len(pairs)} pairs, rows {sum(a.rows for a, _ in pairs)}, "
+          f"nnz(A)+nnz(B) {sum(a.nnz + b.nnz for a, b in pairs)}, products {products}, "
+          f"generated in {t_gen:.1f} s", file=sys.stderr, flush=True)
+    n = len(pairs)
 
-    hip = Hip()
+    def alloc_out(nnz):
+        tot = int(sum(nnz))
+        ci = torch.empty(max(tot, 1), dtype=torch.int32, device="cuda")
+        val = torch.empty(max(tot, 1), dtype=torch.float32, device="cuda")
+        rps = [torch.empty(a.rows + 1, dtype=torch.int32, device="cuda") for a, _ in pairs]
+        structs = (F.CsrMatrix * n)()
+        off = 0
+        for p, (A, B) in enumerate(pairs):
+            structs[p] = F.CsrMatrix(A.rows, B.cols, int(nnz[p]), rps[p].data_ptr(),
+                                     ci.data_ptr() + 4 * off, val.data_ptr() + 4 * off,
+                                     F.DTYPE_FP32, F.FORMAT_CSR, 0)
+            off += int(nnz[p])
+        return ci, val, rps, structs
+
+    a_st = [A.struct() for A, _ in pairs]
+    b_st = [B.struct() for _, B in pairs]
+    # Plain: a descriptor per pair, each call synchronous.
+    descs = []
+    hip.sync()
+    t0 = time.perf_counter()
+    nnz_plain = []
+    for p in range(n):
+        d = rvsp.Descriptor(algo=F.ALGO_HIP_FIXED, memory=F.MEMORY_DEVICE)
+        ws, nz = C.c_size_t(), C.c_int32()
+        F.check(L.rvsp_spgemm_work_estimation(d.handle, C.byref(a_st[p]), C.byref(b_st[p]),
+                                              C.byref(ws), C.byref(nz)), "estimate")
+        nnz_plain.append(nz.value)
+        descs.append(d)
+    t_plain_est = time.perf_counter() - t0
+    ci_p, val_p, rps_p, out_p = alloc_out(nnz_plain)
+    tok = C.c_void_p(descs[0].ws.ctypes.data)
+    hip.sync()
+    t0 = time.perf_counter()
+    for p in range(n):
+        F.check(L.rvsp_spgemm_compute(descs[p].handle, C.byref(a_st[p]), C.byref(b_st[p]),
+                                      C.byref(out_p[p]), tok), "compute")
+    t_plain_first = time.perf_counter() - t0
+    reps = args.batch_repeats
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for p in range(n):
+            F.check(L.rvsp_spgemm_compute(descs[p].handle, C.byref(a_st[p]), C.byref(b_st[p]),
+                                          C.byref(out_p[p]), tok), "compute")
+    t_plain_num = (time.perf_counter() - t0) / reps
+    for d in descs:
+        d.close()
+    print(f"# batched: plain estimate {t_plain_est * 1e3:.1f} ms, first compute "
+          f"{t_plain_first * 1e3:.1f} ms, numeric {t_plain_num * 1e3:.1f} ms",
+          file=sys.stderr, flush=True)
+    sums_plain = (word_checksum(ci_p), word_checksum(val_p),
+                  sum(word_checksum(r) for r in rps_p) % (1 << 64))
+    # Batched.
+    bt = Batch(algo=F.ALGO_HIP_LEARNED if args.batch_algo == "learned" else F.ALGO_HIP_FIXED)
+    hip.sync()
+    t0 = time.perf_counter()
+    As, Bs = [A for A, _ in pairs], [B for _, B in pairs]
+    nnz_b = bt.estimate(As, Bs)
+    t_b_est = time.perf_counter() - t0
+    del ci_p, val_p, rps_p
+    ci_b, val_b, rps_b, out_b = alloc_out(nnz_b)
+    hip.sync()
+    t0 = time.perf_counter()
+    F.check(bt.compute(As, Bs, out_b), "batch compute")
+    t_b_first = time.perf_counter() - t0
+    F.check(bt.compute(As, Bs, out_b), "batch compute")  # captured here
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        F.check(bt.compute(As, Bs, out_b), "batch compute")
+    t_b_num = (time.perf_counter() - t0) / reps
+    groups, graphs = bt.groups(), bt.graph_launches()
+    bt.close()
+    sums_b = (word_checksum(ci_b), word_checksum(val_b),
+              sum(word_checksum(r) for r in rps_b) % (1 << 64))
+    same = list(nnz_b) == nnz_plain and sums_b == sums_plain
+    # Oracle on a few pairs.
+    oracle_ok = True
+    for p in range(0, n, max(1, n // 4)):
+        A, B = pairs[p][0].to_host(), pairs[p][1].to_host()
+        rp, ci, v = oracle.spgemm(A, B)
+        o = out_b[p]
+        k = int(nnz_b[p])
+        got_rp = rps_b[p].cpu().numpy()
+        got_ci = torch.tensor([], dtype=torch.int32) if k == 0 else \
+            ci_b[(o.col_idx - ci_b.data_ptr()) // 4:(o.col_idx - ci_b.data_ptr()) // 4 + k].cpu()
+        got_v = val_b[(o.values - val_b.data_ptr()) // 4:(o.values - val_b.data_ptr()) // 4 + k].cpu()
+        oracle_ok &= (got_rp.tobytes() == rp.tobytes() and got_ci.numpy().tobytes() == ci.tobytes()
+                      and got_v.numpy().tobytes() == v.tobytes())
+    cpu = None
+    if not args.no_cpu:
+        host_pairs = ((A.to_host(), B.to_host()) for A, B in pairs)
+        spent, prods, npairs, kind = cpu_oneshot(host_pairs, args.cpu_seconds / 2)
+        cpu = {"value": round(2 * prods / spent / 1e9, 4), "unit": "GFLOP/s", "cores": 1,
+               "kind": kind, "sample": f"one-shot products of the first {npairs} pairs"}
+    del ci_b, val_b, rps_b, keep
+    torch.cuda.empty_cache()
+    flops = 2.0 * products
+    oneshot_plain = t_plain_est + t_plain_first
+    oneshot_b = t_b_est + t_b_first
+    return {
+        "workload": f"lognormal_pairs{n}_n256-65536",
+        "pairs": n, "products": products, "nnz_c": int(sum(nnz_b)), "groups": groups,
+        "algo_batched": args.batch_algo,
+        "oneshot_ms": {"plain": round(oneshot_plain * 1e3, 2), "batched": round(oneshot_b * 1e3, 2)},
+        "oneshot_gflops": {"plain": round(flops / oneshot_plain / 1e9, 2),
+                           "batched": round(flops / oneshot_b / 1e9, 2)},
+        "speedup_oneshot": round(oneshot_plain / oneshot_b, 3),
+        "numeric_ms": {"plain": round(t_plain_num * 1e3, 3), "batched_graph": round(t_b_num * 1e3, 3)},
+        "numeric_gflops": {"plain": round(flops / t_plain_num / 1e9, 2),
+                           "batched_graph": round(flops / t_b_num / 1e9, 2)},
+        "speedup_numeric": round(t_plain_num / t_b_num, 3),
+        "graph_launches": graphs,
+        "verified_bit_exact": bool(same and oracle_ok),
+        "cpu_baseline": cpu,
+    }
+
+
+def row_blocks(A, max_products):
+    """Row cuts of A so that every block of C = A*A has at most
+    max_products products."""
+    lens = np.diff(A.row_ptr).astype(np.int64)
+    cs = np.concatenate([[0], np.cumsum(lens[A.col_idx])])
+    ops = cs[A.row_ptr[1:]] - cs[A.row_ptr[:-1]]
+    cuts, acc = [0], 0
+    for i, o in enumerate(ops):
+        if acc + o > max_products and i > cuts[-1]:
+            cuts.append(i)
+            acc = 0
+        acc += int(o)
+    cuts.append(A.rows)
+    return cuts, ops
+
+
+def bench_powerlaw(hip, args, F):
+    """R-MAT C = A*A: nnz(C) exceeds int32, so the product runs as row
+    blocks of A against the same B. Plain: one FIXED descriptor per block,
+    one after another. Learned + batched: all blocks as pairs of one batched
+    HIP_LEARNED descriptor."""
+    import torch
+    import rvsp
+    from oracle import oracle
+    from rvsp import gen
+    from rvsp.batch import Batch
+    L = F.lib()
+    t0 = time.perf_counter()
+    A = gen.rmat(args.rmat_scale, seed=0)
+    cuts, ops = row_blocks(A, 1 << 30)
+    products = int(ops.sum())
+    print(f"# powerlaw: R-MAT 2^{args.rmat_scale} nnz {A.nnz}, products {products}, "
+          f"{len(cuts) - 1} row blocks, built in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    dev = "cuda"
+    rp = torch.as_tensor(A.row_ptr.astype(np.int32), device=dev)
+    ci = torch.as_tensor(A.col_idx.astype(np.int32), device=dev)
+    val = torch.as_tensor(A.values, device=dev)
+    full = F.CsrMatrix(A.rows, A.cols, A.nnz, rp.data_ptr(), ci.data_ptr(), val.data_ptr(),
+                       F.DTYPE_FP32, F.FORMAT_CSR, 0)
+    blocks, brps = [], []
+    for r0, r1 in zip(cuts[:-1], cuts[1:]):
+        brp = (rp[r0:r1 + 1] - rp[r0]).contiguous()
+        brps.append(brp)
+        o = int(A.row_ptr[r0])
+        blocks.append(F.CsrMatrix(r1 - r0, A.cols, int(A.row_ptr[r1]) - o, brp.data_ptr(),
+                                  ci.data_ptr() + 4 * o, val.data_ptr() + 4 * o,
+                                  F.DTYPE_FP32, F.FORMAT_CSR, 0))
+    nb = len(blocks)
+
+    # Plain (FIXED): per block, estimate then compute, one block after
+    # another, each into its own output (allocation untimed).
+    nnz_blocks = []
+    t_plain = 0.0
+    c_rps = [torch.empty(b.rows + 1, dtype=torch.int32, device=dev) for b in blocks]
+    sums_plain = [0, 0, 0]
+    word_off = 0
+    for p in range(nb):
+        d = rvsp.Descriptor(algo=F.ALGO_HIP_FIXED, memory=F.MEMORY_DEVICE)
+        ws, nz = C.c_size_t(), C.c_int32()
+        hip.sync()
+        t0 = time.perf_counter()
+        F.check(L.rvsp_spgemm_work_estimation(d.handle, C.byref(blocks[p]), C.byref(full),
+                                              C.byref(ws), C.byref(nz)), "estimate")
+        t_plain += time.perf_counter() - t0
+        k = nz.value
+        nnz_blocks.append(k)
+        bci = torch.empty(max(k, 1), dtype=torch.int32, device=dev)
+        bval = torch.empty(max(k, 1), dtype=torch.float32, device=dev)
+        cm = F.CsrMatrix(blocks[p].rows, A.cols, k, c_rps[p].data_ptr(), bci.data_ptr(),
+                         bval.data_ptr(), F.DTYPE_FP32, F.FORMAT_CSR, 0)
+        hip.sync()
+        t0 = time.perf_counter()
+        F.check(L.rvsp_spgemm_compute(d.handle, C.byref(blocks[p]), C.byref(full), C.byref(cm),
+                                      C.c_void_p(d.ws.ctypes.data)), "compute")
+        t_plain += time.perf_counter() - t0
+        d.close()
+        sums_plain[0] += word_checksum(bci[:k], word_off)
+        sums_plain[1] += word_checksum(bval[:k], word_off)
+        sums_plain[2] += word_checksum(c_rps[p])
+        word_off += k
+        del bci, bval
+    sums_plain = tuple(x % (1 << 64) for x in sums_plain)
+    total = int(sum(nnz_blocks))
+    torch.cuda.empty_cache()
+    ci_out = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+    val_out = torch.empty(max(total, 1), dtype=torch.float32, device=dev)
+    structs = (F.CsrMatrix * nb)()
+    off = 0
+    for p in range(nb):
+        structs[p] = F.CsrMatrix(blocks[p].rows, A.cols, nnz_blocks[p], c_rps[p].data_ptr(),
+                                 ci_out.data_ptr() + 4 * off, val_out.data_ptr() + 4 * off,
+                                 F.DTYPE_FP32, F.FORMAT_CSR, 0)
+        off += nnz_blocks[p]
+    print(f"# powerlaw: plain fixed {t_plain * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    # Learned + batched.
+    results = {}
+    for name, algo in (("batched_fixed", F.ALGO_HIP_FIXED), ("learned_batched", F.ALGO_HIP_LEARNED)):
+        bt = Batch(algo=algo)
+        hip.sync()
+        t0 = time.perf_counter()
+        nnz_b = bt.estimate(list(blocks), [full] * nb)
+        F.check(bt.compute(list(blocks), [full] * nb, structs), "batch compute")
+        t_b = time.perf_counter() - t0
+        groups = bt.groups()
+        bt.close()
+        sums_b = (word_checksum(ci_out[:total]), word_checksum(val_out[:total]),
+                  sum(word_checksum(r) for r in c_rps) % (1 << 64))
+        results[name] = {"ms": t_b * 1e3, "groups": groups,
+                         "same": list(nnz_b) == nnz_blocks and sums_b == sums_plain}
+        print(f"# powerlaw: {name} {t_b * 1e3:.1f} ms ({groups} groups)", file=sys.stderr,
+              flush=True)
+    # Oracle on four random ranges of 64 rows.
+    oracle_ok = True
+    rng = np.random.default_rng(1)
+    starts = rng.integers(0, A.rows - 64, size=4)
+    off_blk = np.concatenate([[0], np.cumsum(nnz_blocks)])
+    for r0 in starts:
+        b = int(np.searchsorted(cuts, r0, side="right") - 1)
+        r1 = min(int(r0) + 64, cuts[b + 1])
+        want_rp, want_ci, want_v = oracle.spgemm(A, A, int(r0), r1)
+        crp = c_rps[b].cpu().numpy()
+        lo, hi = int(r0) - cuts[b], r1 - cuts[b]
+        s, e = int(crp[lo]), int(crp[hi])
+        got_ci = ci_out[off_blk[b] + s:off_blk[b] + e].cpu().numpy()
+        got_v = val_out[off_blk[b] + s:off_blk[b] + e].cpu().numpy()
+        oracle_ok &= (np.array_equal(crp[lo:hi + 1] - s, want_rp) and
+                      got_ci.tobytes() == want_ci.tobytes() and got_v.tobytes() == want_v.tobytes())
+    cpu = None
+    if not args.no_cpu:
+        # Reference one-shot on whole row blocks of 256 rows spread over A.
+        sample_rows = np.linspace(0, A.rows - 256, 24).astype(np.int64)
+        spent, prods = 0.0, 0
+        from rvsp.api import Csr
+        kind = "reference" if oracle.ref_available() else "port"
+        for r0 in sample_rows:
+            r1 = int(r0) + 256
+            o0, o1 = A.row_ptr[r0], A.row_ptr[r1]
+            blk = Csr(256, A.cols, A.row_ptr[r0:r1 + 1] - o0, A.col_idx[o0:o1], A.values[o0:o1])
+            t0 = time.perf_counter()
+            if kind == "reference":
+                oracle.ref_spgemm(blk, A)
+            else:
+                oracle.spgemm(blk, A)
+            spent += time.perf_counter() - t0
+            prods += int(ops[r0:r1].sum())
+            if spent > args.cpu_seconds / 2:
+                break
+        cpu = {"value": round(2 * prods / spent / 1e9, 4), "unit": "GFLOP/s", "cores": 1,
+               "kind": kind, "sample": "one-shot products of 256-row blocks spread over A"}
+    del ci_out, val_out, c_rps
+    torch.cuda.empty_cache()
+    flops = 2.0 * products
+    lb = results["learned_batched"]
+    return {
+        "workload": f"rmat_s{args.rmat_scale}_ef16_AxA_rowblocks",
+        "products": products, "nnz_c": total, "row_blocks": nb,
+        "oneshot_ms": {"plain_fixed": round(t_plain * 1e3, 1),
+                       "batched_fixed": round(results["batched_fixed"]["ms"], 1),
+                       "learned_batched": round(lb["ms"], 1)},
+        "gflops": {"plain_fixed": round(flops / t_plain / 1e9, 2),
+                   "learned_batched": round(flops / (lb["ms"] * 1e-3) / 1e9, 2)},
+        "speedup_learned_batched_vs_plain": round(t_plain * 1e3 / lb["ms"], 3),
+        "verified_bit_exact": bool(all(r["same"] for r in results.values()) and oracle_ok),
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_banded(hip, args, F):
+    """A 500k x 200k banded (24 of 32) with every 100th row dense-ish (2000),
+    B 200k x 500k uniform 8 per row: FIXED vs LEARNED, one-shot and
+    numeric-only."""
+    import rvsp
+    from oracle import oracle
+    from rvsp import gen
+    L = F.lib()
+    A, B = gen.banded_mixed(seed=0)
+    products = int(np.diff(B.row_ptr)[A.col_idx].sum())
+    print(f"# banded: A {A.rows}x{A.cols} nnz {A.nnz}, B nnz {B.nnz}, products {products}",
+          file=sys.stderr, flush=True)
+    dA, dB = rvsp.DeviceCsr.upload(A), rvsp.DeviceCsr.upload(B)
+    res = {}
+    outs = {}
+    for name, algo in (("fixed", F.ALGO_HIP_FIXED), ("learned", F.ALGO_HIP_LEARNED)):
+        d = rvsp.Descriptor(algo=algo, memory=F.MEMORY_DEVICE)
+        hip.sync()
+        t0 = time.perf_counter()
+        nnz = d.estimate(dA, dB)
+        dC = rvsp.DeviceCsr(A.rows, B.cols, nnz)
+        cm = dC.struct(nnz)
+        t_alloc = time.perf_counter()
+        F.check(d.compute(dA, dB, cm), "compute")
+        t1 = time.perf_counter()
+        oneshot = (t_alloc - t0) + (t1 - t_alloc)
+        hip.sync()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            F.check(d.compute(dA, dB, cm), "compute")
+        num = (time.perf_counter() - t0) / 5
+        d.close()
+        res[name] = {"oneshot_ms": oneshot * 1e3, "numeric_ms": num * 1e3}
+        outs[name] = dC.download()
+        dC.free()
+    same = all(getattr(outs["fixed"], k).tobytes() == getattr(outs["learned"], k).tobytes()
+               for k in ("row_ptr", "col_idx", "values"))
+    rp, ci, v = oracle.spgemm(A, B, 0, 5000)
+    got = outs["fixed"]
+    ok = (np.array_equal(got.row_ptr[:5001], rp) and got.col_idx[:rp[-1]].tobytes() == ci.tobytes()
+          and got.values[:rp[-1]].tobytes() == v.tobytes())
+    dA.free()
+    dB.free()
+    flops = 2.0 * products
+    return {
+        "workload": "banded500k_x_uniform200k_mixed_dense_rows",
+        "products": products, "nnz_c": int(outs["fixed"].nnz),
+        "oneshot_ms": {k: round(r["oneshot_ms"], 2) for k, r in res.items()},
+        "numeric_ms": {k: round(r["numeric_ms"], 3) for k, r in res.items()},
+        "numeric_gflops": {k: round(flops / (r["numeric_ms"] * 1e-3) / 1e9, 2) for k, r in res.items()},
+        "speedup_learned_vs_fixed_oneshot": round(res["fixed"]["oneshot_ms"] / res["learned"]["oneshot_ms"], 3),
+        "verified_bit_exact": bool(same and ok),
+    }
+
+
+def bench_headline(hip, args, F):
+    """The headline: numeric-only multiplies of the Laplacian product."""
+    import rvsp
+    from rvsp import gen
     A = gen.laplacian_2d(args.grid)
     n_vals = args.steps + args.warmup
     print(f"# laplacian5pt g={args.grid}: rows={A.rows} nnz={A.nnz}; staging {n_vals} "
@@ -273,12 +645,77 @@ def main():
                       "autotune": round(head["t_tune"] * 1e3, 1) if head["t_tune"] else None},
         "verified_bit_exact": verified,
     }
-    print(json.dumps(out), flush=True)
     for r in runs.values():
         r["C"].free()
     for p in val_bufs:
         rvsp.device_free(p)
     dA.free()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--grid", type=int, default=1024)
+    ap.add_argument("--autotune-reps", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--only", choices=["fixed", "tuned"], default=None,
+                    help="time one plan only (profiling runs)")
+    ap.add_argument("--extra", default="batched,powerlaw,banded",
+                    help="secondary workloads reported under 'configs' (comma list, or none)")
+    ap.add_argument("--only-extra", action="store_true",
+                    help="skip the headline workload (profiling runs of the extras)")
+    ap.add_argument("--pairs", type=int, default=2048)
+    ap.add_argument("--batch-repeats", type=int, default=5)
+    ap.add_argument("--batch-algo", choices=["fixed", "learned"], default="learned")
+    ap.add_argument("--rmat-scale", type=int, default=20)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run device-to-device copies of this many MiB (a known "
+                         "byte count for scaling the HBM traffic counters), then exit")
+    args = ap.parse_args()
+    if args.calibrate_copy_mib:
+        import rvsp
+        n = args.calibrate_copy_mib << 20
+        src, dst = rvsp.device_alloc(n), rvsp.device_alloc(n)
+        for _ in range(3):
+            rvsp._ffi.check(rvsp.lib().rvsp_hip_memcpy(C.c_void_p(dst), C.c_void_p(src), n, 3))
+        rvsp.synchronize()
+        print(json.dumps({"calibration_copy_bytes": n, "copies": 3}))
+        return
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has one MI355X, and "
+              "the SpGEMM path has no multi-GPU mode (DESIGN.md describes how it would "
+              "shard by output rows).", file=sys.stderr)
+        sys.exit(2)
+
+    extra = [x for x in args.extra.split(",") if x and x != "none"]
+    if extra:
+        # The secondary workloads build operands with torch. torch brings its
+        # own HIP runtime; loading it first makes the library bind to that
+        # same runtime (one runtime per process).
+        import torch
+        torch.zeros(1, device="cuda")
+    import rvsp
+    from rvsp import _ffi as F
+
+    hip = Hip()
+    out = {} if args.only_extra else bench_headline(hip, args, F)
+    extras = {"batched": bench_batched, "powerlaw": bench_powerlaw, "banded": bench_banded}
+    configs = {}
+    for name in extra:
+        t0 = time.perf_counter()
+        try:
+            configs[name] = extras[name](hip, args, F)
+        except Exception as e:  # reported, never hidden: the line says which failed
+            configs[name] = {"error": f"{type(e).__name__}: {e}"}
+        configs[name]["wall_s"] = round(time.perf_counter() - t0, 1)
+        print(f"# {name}: {json.dumps(configs[name])}", file=sys.stderr, flush=True)
+    out["configs"] = configs
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
