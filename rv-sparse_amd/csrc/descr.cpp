@@ -389,7 +389,7 @@ rvsp_status_t run_numeric(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B
                                           c_val, s));
     } else {
       RVSP_HIP_TRY(launch_numeric_bin(spec, rows, n, A, B, st, (const int*)d->c_rp.p, c_ci,
-                                      c_val, d->gs_dense, s));
+                                      c_val, d->gs_dense, d->numeric.table.rule.bound[b], s));
     }
   }
   return RVSP_SUCCESS;

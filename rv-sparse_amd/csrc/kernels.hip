@@ -1377,6 +1377,126 @@ __global__ __launch_bounds__(kBlock) void k_numeric_rowblock(
   }
 }
 
+// Long rows: one workgroup per (row, tile of T consecutive output
+// positions). The tile's columns and accumulators live in LDS. The row's
+// products are streamed in reference order (A entries ascending, then B
+// positions ascending) in groups of one product per lane: every lane finds
+// its product's A entry, gathers the B element and looks its column up in
+// the tile, all lanes in parallel. Only the accumulate is ordered: products
+// of a group that land on the same position are applied in product order,
+// the lowest lane first (an LDS atomicMin picks it per round), so every
+// C(i,j) sees exactly the reference's sequence of separately rounded adds.
+// Tiles of one row share nothing but the A row and B rows they read; a row
+// spanning several tiles first narrows long B rows to the tile's columns.
+struct TiledLds {
+  int* cols;
+  float* vals;
+  int* tag;
+  int* pre;    // kBlock + 1 exclusive prefix of the chunk's product counts
+  int* bstart; // kBlock B-row starts
+  float* aval; // kBlock A values
+};
+
+__device__ __forceinline__ TiledLds tiled_lds(int* lds, int T) {
+  TiledLds L;
+  L.cols = lds;
+  L.vals = reinterpret_cast<float*>(lds + T);
+  L.tag = lds + 2 * T;
+  L.pre = lds + 3 * T;
+  L.bstart = L.pre + kBlock + 1;
+  L.aval = reinterpret_cast<float*>(L.bstart + kBlock);
+  return L;
+}
+
+__global__ __launch_bounds__(kBlock) void k_numeric_tiled(
+    const int* __restrict__ rows, int nrows, int T, DevCsr A, DevCsr B,
+    const int* __restrict__ c_rp, const int* __restrict__ c_ci,
+    float* __restrict__ c_val) {
+  extern __shared__ int lds[];
+  __shared__ int scan_tmp[kBlock / kWave];
+  const TiledLds L = tiled_lds(lds, T);
+  const int tid = threadIdx.x;
+  for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
+    const int row = rows[r];
+    const int c_s = c_rp[row];
+    const int n = c_rp[row + 1] - c_s;
+    const int ntiles = (n + T - 1) / T;
+    for (int t = blockIdx.y; t < ntiles; t += gridDim.y) {
+      const int t0 = t * T;
+      const int tn = min(T, n - t0);
+      for (int j = tid; j < tn; j += kBlock) {
+        L.cols[j] = c_ci[c_s + t0 + j];
+        L.vals[j] = 0.0f;
+        L.tag[j] = INT_MAX;
+      }
+      __syncthreads();
+      const int clo = L.cols[0];
+      const int chi = L.cols[tn - 1];
+      const bool narrow = ntiles > 1;
+      const int a_e = A.rp[row + 1];
+      for (int cb = A.rp[row]; cb < a_e; cb += kBlock) {
+        // One A entry per lane: its scale and the part of its B row that
+        // can land in this tile.
+        const int ap = cb + tid;
+        int bs = 0, len = 0;
+        float a = 0.0f;
+        if (ap < a_e) {
+          const int k = A.ci[ap];
+          a = A.val[ap];
+          bs = B.rp[k];
+          int be = B.rp[k + 1];
+          if (narrow && be - bs > 32) {
+            bs = gm_lower_bound(B.ci, bs, be, clo);
+            be = gm_lower_bound(B.ci, bs, be, chi + 1);
+          }
+          len = be - bs;
+        }
+        int total;
+        const int excl = block_excl_scan(len, scan_tmp, &total);
+        L.pre[tid] = excl;
+        L.bstart[tid] = bs;
+        L.aval[tid] = a;
+        __syncthreads();
+        for (int q0 = 0; q0 < total; q0 += kBlock) {
+          const int q = q0 + tid;
+          int pos = -1;
+          float prod = 0.0f;
+          if (q < total) {
+            // The A entry of product q: the last e with pre[e] <= q (entries
+            // without products share their successor's prefix).
+            int e = 0;
+#pragma unroll
+            for (int step = kBlock / 2; step > 0; step >>= 1)
+              if (L.pre[e + step] <= q) e += step;
+            const int bi = L.bstart[e] + (q - L.pre[e]);
+            const int col = B.ci[bi];
+            const float bv = B.val[bi];
+            if (col >= clo && col <= chi) {
+              pos = lds_find(L.cols, tn, col);
+              prod = L.aval[e] * bv;  // rounded here, added below
+            }
+          }
+          bool active = pos >= 0;
+          while (__syncthreads_or(active)) {
+            if (active) atomicMin(&L.tag[pos], tid);
+            __syncthreads();
+            if (active && L.tag[pos] == tid) {
+              L.vals[pos] = L.vals[pos] + prod;
+              L.tag[pos] = INT_MAX;
+              active = false;
+            }
+          }
+        }
+        __syncthreads();  // the chunk's metadata is replaced next
+      }
+      for (int j = tid; j < tn; j += kBlock) c_val[c_s + t0 + j] = L.vals[j];
+      __syncthreads();  // the tile's LDS is reused by the next tile
+    }
+  }
+}
+
+int tiled_lds_bytes(int T) { return (3 * T + 3 * kBlock + 1) * 4; }
+
 template <int G>
 static hipError_t launch_numeric_g(const BinSpec& b, const int* rows, int nrows,
                                    const DevCsr& A, const DevCsr& B,
@@ -1398,7 +1518,7 @@ static hipError_t launch_numeric_g(const BinSpec& b, const int* rows, int nrows,
 hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
                               const DevCsr& A, const DevCsr& B, RowStats st,
                               const int* c_rp, const int* c_ci, float* c_val,
-                              const GlobalSlots& gs, hipStream_t s) {
+                              const GlobalSlots& gs, int max_row, hipStream_t s) {
   (void)st;
   if (nrows <= 0) return hipSuccess;
   switch (b.kind) {
@@ -1414,11 +1534,16 @@ hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
         case 64: return launch_numeric_g<64>(b, rows, nrows, A, B, c_rp, c_ci, c_val, s);
         default: return hipErrorInvalidValue;
       }
-    case KIND_BLOCK:
-      hipLaunchKernelGGL(k_numeric_rowblock<false>, dim3(row_grid(nrows)),
-                         dim3(kBlock), b.lds_bytes, s, rows, nrows, b.cap, A, B,
-                         c_rp, c_ci, c_val, nullptr, (size_t)0);
+    case KIND_BLOCK: {
+      // Tiles of one row get the same x, so they share an XCD (and its L2)
+      // when the x extent is a multiple of the XCD count.
+      const long long most = max_row > 0 ? max_row : 1;
+      const int ytiles = (int)std::min<long long>(16, (most + b.cap - 1) / b.cap);
+      const int gx = (std::min(nrows, 1 << 20) + 7) / 8 * 8;
+      hipLaunchKernelGGL(k_numeric_tiled, dim3(gx, ytiles), dim3(kBlock), b.lds_bytes, s,
+                         rows, nrows, b.cap, A, B, c_rp, c_ci, c_val);
       return hipGetLastError();
+    }
     case KIND_GLOBAL:
       if (!gs.base || gs.nslots <= 0) return hipErrorInvalidValue;
       hipLaunchKernelGGL(k_numeric_rowblock<true>, dim3(min(nrows, gs.nslots)),
@@ -1455,7 +1580,7 @@ hipError_t init_kernel_attributes(int lds_bytes) {
   RVSP_ALLOW(k_numeric_search<16>) RVSP_ALLOW(k_numeric_search<32>)
   RVSP_ALLOW(k_numeric_search<64>)
   RVSP_ALLOW((k_bitmap_row<false, false>)) RVSP_ALLOW((k_bitmap_row<false, true>))
-  RVSP_ALLOW(k_numeric_rowblock<false>)
+  RVSP_ALLOW(k_numeric_rowblock<false>) RVSP_ALLOW(k_numeric_tiled)
 #define RVSP_ALLOW_MAP(G)                                                     \
   RVSP_ALLOW((k_numeric_map<G, unsigned char, 0>))                             \
   RVSP_ALLOW((k_numeric_map<G, unsigned char, 1>))                             \

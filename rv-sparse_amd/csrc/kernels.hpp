@@ -42,7 +42,7 @@ enum Kind : int {
   KIND_HASH = 2,     // LDS hash table per row, G lanes per row (count, fill)
   KIND_SEARCH = 3,   // LDS sorted columns + binary search, G lanes per row (numeric)
   KIND_BITMAP = 4,   // one workgroup per row, LDS bitmap over the row's span (count, fill)
-  KIND_BLOCK = 5,    // one workgroup per row, LDS sorted columns (numeric)
+  KIND_BLOCK = 5,    // one workgroup per tile of a long row, LDS columns (numeric)
   KIND_GLOBAL = 6,   // one workgroup per row, dense scratch in global memory
   KIND_MAP = 7       // position map + LDS accumulators, G lanes per row (numeric)
 };
@@ -54,8 +54,8 @@ constexpr int kMaxBins = 24;
 // Row -> bin rule. Bin 0 is reserved for rows that need no table (empty or
 // single-A-entry rows). Other rows go to the first bin b >= 1 whose
 // `bound[b]` >= the row's size metric and whose `span_max[b]` >= the row's
-// column span; rows matching none go to the last bin, which must be a
-// GLOBAL bin. The size metric is the distinct-column upper bound
+// column span; rows matching none go to the last bin, whose kind must take
+// rows of any length (GLOBAL, or tiled BLOCK for numeric). The size metric is the distinct-column upper bound
 // min(Op_i, span_i) for the count phase and the exact nnz of the C row for
 // fill and numeric.
 struct BinRule {
@@ -69,7 +69,7 @@ struct BinSpec {
   int kind = KIND_NONE;
   int group = 1;     // lanes per row (power of two <= 64, or 256 for block kinds)
   int cap = 0;       // per-row LDS capacity: table slots (HASH), columns
-                     // (SEARCH/BLOCK), bitmap bits (BITMAP), span (DENSE)
+                     // (SEARCH, MAP), positions per tile (BLOCK), bitmap bits (BITMAP)
   int lds_bytes = 0; // dynamic LDS per workgroup
   int variant = 0;   // kernel variant of the kind (MAP: 1 = A metadata staged in LDS)
 };
@@ -129,10 +129,15 @@ hipError_t launch_fill_bin(const BinSpec& b, const int* rows, int nrows,
                            hipStream_t s);
 
 // --- numeric ------------------------------------------------------------
+// max_row: the largest C-row length the bin admits (its bound); KIND_BLOCK
+// sizes its tile grid from it (tiles of b.cap positions).
 hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
                               const DevCsr& A, const DevCsr& B, RowStats st,
                               const int* c_rp, const int* c_ci, float* c_val,
-                              const GlobalSlots& gs, hipStream_t s);
+                              const GlobalSlots& gs, int max_row, hipStream_t s);
+
+// Dynamic LDS of the tiled long-row numeric kernel for tiles of T positions.
+int tiled_lds_bytes(int T);
 
 // --- position map (numeric structure cache) -------------------------------
 struct MapView {

@@ -159,7 +159,7 @@ BinChoice class_choice(int cls, int phase, int b, int max_metric) {
       ch.cap = fs.kind == KIND_SEARCH ? fs.cap : next_pow2(std::max(16LL, need));
       break;
     case KIND_BLOCK:
-      ch.cap = fs.kind == KIND_BLOCK ? fs.cap : (need <= 4096 ? 4096 : 16384);
+      ch.cap = 4096;  // positions per tile
       break;
     case KIND_BITMAP:
       ch.cap = fs.cap;

@@ -10,6 +10,8 @@ namespace rvsp {
 
 namespace {
 
+constexpr int kTile = 4096;  // positions per tile of the long-row numeric kernel
+
 int g_lds_limit = kMaxLdsBytes - 8 * 1024;  // dynamic part, set per device
 
 bool pow2(int x) { return x > 0 && (x & (x - 1)) == 0; }
@@ -80,8 +82,10 @@ int lds_bytes_for(int phase, int kind, int group, int cap, int variant) {
       bytes = cap / 8;
       break;
     case KIND_BLOCK:
-      if (phase != PHASE_NUMERIC || cap < 1) break;
-      bytes = (long long)cap * 8;
+      // Tiles of `cap` positions: columns, accumulators and order tags,
+      // plus one chunk of A-entry metadata.
+      if (phase != PHASE_NUMERIC || cap < kBlock) break;
+      bytes = tiled_lds_bytes(cap);
       break;
     default:
       break;
@@ -134,9 +138,10 @@ Strategy fixed_strategy() {
     const int G[] = {4, 8, 16, 32, 64, 64, 64, 64};
     for (int i = 0; i < 8; ++i)
       set_bin(t, PHASE_NUMERIC, b++, KIND_SEARCH, G[i], C[i], C[i], INT_MAX);
-    set_bin(t, PHASE_NUMERIC, b++, KIND_BLOCK, kBlock, 4096, 4096, INT_MAX);
-    set_bin(t, PHASE_NUMERIC, b++, KIND_BLOCK, kBlock, 16384, 16384, INT_MAX);
-    set_bin(t, PHASE_NUMERIC, b++, KIND_GLOBAL, kBlock, 0, INT_MAX, INT_MAX);
+    // Longer rows: tiles of 4096 positions, one workgroup per tile.
+    set_bin(t, PHASE_NUMERIC, b++, KIND_BLOCK, kBlock, kTile, 4096, INT_MAX);
+    set_bin(t, PHASE_NUMERIC, b++, KIND_BLOCK, kBlock, kTile, 16384, INT_MAX);
+    set_bin(t, PHASE_NUMERIC, b++, KIND_BLOCK, kBlock, kTile, INT_MAX, INT_MAX);
     t.rule.nbins = b;
   }
   return s;
@@ -165,7 +170,8 @@ bool apply_choice(PhaseTable& t, int phase, int b, const BinChoice& c,
         ok = phase == PHASE_NUMERIC && max_metric <= cand.cap;
         break;
       case KIND_BLOCK:
-        ok = phase == PHASE_NUMERIC && max_metric <= cand.cap;
+        // Tiled: any row length.
+        ok = phase == PHASE_NUMERIC;
         if (ok) cand.group = kBlock;
         break;
       case KIND_BITMAP:

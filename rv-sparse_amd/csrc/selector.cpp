@@ -145,7 +145,7 @@ namespace {
 // Launches one bin's kernel of `phase` with shape `spec`.
 hipError_t launch_bin(rvsp_spgemm_descr* d, int phase, const BinSpec& spec, const int* rows,
                       int n, const DevCsr& A, const DevCsr& B, int* c_ci, float* c_val,
-                      hipStream_t s) {
+                      int max_row, hipStream_t s) {
   const RowStats st = row_stats(d);
   switch (phase) {
     case PHASE_COUNT:
@@ -162,7 +162,7 @@ hipError_t launch_bin(rvsp_spgemm_descr* d, int phase, const BinSpec& spec, cons
         return launch_numeric_map_bin(spec, mv, rows, n, A, B, (const int*)d->c_rp.p, c_val, s);
       }
       return launch_numeric_bin(spec, rows, n, A, B, st, (const int*)d->c_rp.p, c_ci, c_val,
-                                d->gs_dense, s);
+                                d->gs_dense, max_row, s);
   }
 }
 
@@ -229,10 +229,11 @@ rvsp_status_t autotune_plan(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr&
         const GlobalSlots& gs = ph == PHASE_NUMERIC ? d->gs_dense : d->gs_bits;
         if (spec.kind == KIND_GLOBAL && gs.nslots == 0) continue;  // no scratch sized
         // One untimed launch, then `reps` timed ones.
-        hipError_t e = launch_bin(d, ph, spec, rows, n, A, B, c_ci, c_val, s);
+        const int max_row = p.table.rule.bound[b];
+        hipError_t e = launch_bin(d, ph, spec, rows, n, A, B, c_ci, c_val, max_row, s);
         if (e == hipSuccess) e = hipEventRecord(e0, s);
         for (int r = 0; r < reps && e == hipSuccess; ++r)
-          e = launch_bin(d, ph, spec, rows, n, A, B, c_ci, c_val, s);
+          e = launch_bin(d, ph, spec, rows, n, A, B, c_ci, c_val, max_row, s);
         if (e == hipSuccess) e = hipEventRecord(e1, s);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.f;

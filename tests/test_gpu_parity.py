@@ -95,7 +95,11 @@ def test_random_shapes_bit_exact(gpu, seed):
 # the same bits; impossible choices fall back to global dense.
 PLAN_VARIANTS = {
     "all_global": [(ph, b, F.KIND_GLOBAL, 0, 0) for ph in (0, 1, 2) for b in range(2, 24)],
-    "numeric_block": [(2, b, F.KIND_BLOCK, 256, 16384) for b in range(2, 12)],
+    "numeric_block": [(2, b, F.KIND_BLOCK, 256, 4096) for b in range(2, 12)],
+    # Small tiles: rows span many tiles (more than the 16 of the launch grid),
+    # so long B rows are narrowed and tiles loop.
+    "numeric_block_t256": [(2, b, F.KIND_BLOCK, 256, 256) for b in range(2, 13)],
+    "numeric_block_t1024": [(2, b, F.KIND_BLOCK, 256, 1024) for b in range(2, 13)],
     "numeric_search_g1": [(2, b, F.KIND_SEARCH, 1, 0) for b in range(2, 6)],
     "numeric_search_g64": [(2, b, F.KIND_SEARCH, 64, 0) for b in range(2, 10)],
     "hash_g2": [(ph, b, F.KIND_HASH, 2, 0) for ph in (0, 1) for b in range(2, 7)],
@@ -143,7 +147,7 @@ def test_position_map_repeated_numeric(gpu, name):
 
 @pytest.mark.parametrize("variant", sorted(PLAN_VARIANTS))
 @pytest.mark.parametrize("name", ["cfg4_pair_0", "long_rows", "rect_empty_rows",
-                                  "cfg2_laplacian_g64_stencil"])
+                                  "cfg2_laplacian_g64_stencil", "cfg3_rmat_s12"])
 def test_forced_plans_identical_bits(gpu, name, variant):
     A, B = cases.build(name)
     got, plan = rvsp.spgemm_device(A, B, choices=PLAN_VARIANTS[variant])
