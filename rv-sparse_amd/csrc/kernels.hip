@@ -66,6 +66,19 @@ __device__ __forceinline__ void group_for_each_col(int a_s, int a_e,
   }
 }
 
+// The last entry e >= j of a kBlock-long nondecreasing prefix array with
+// pre[e] <= p, given pre[j] <= p: gallop forward, then halve.
+__device__ __forceinline__ int advance_entry(const int* pre, int j, int p) {
+  int step = 1;
+  while (j + step < kBlock && pre[j + step] <= p) {
+    j += step;
+    step <<= 1;
+  }
+  for (step >>= 1; step > 0; step >>= 1)
+    if (j + step < kBlock && pre[j + step] <= p) j += step;
+  return j;
+}
+
 // Whole-workgroup version of group_for_each_col (blockDim.x == kBlock).
 // `sh_excl` and `sh_bs` hold kBlock ints each, `tmp` kBlock / kWave ints.
 // Must be reached by every thread of the block.
@@ -90,12 +103,21 @@ __device__ __forceinline__ void block_for_each_col(int a_s, int a_e,
     sh_excl[t] = excl;
     sh_bs[t] = bs;
     __syncthreads();
-    for (int p = t; p < total; p += kBlock) {
-      int j = 0;
-#pragma unroll
-      for (int step = kBlock >> 1; step > 0; step >>= 1)
-        if (sh_excl[j + step] <= p) j += step;
-      f(b_ci[sh_bs[j] + (p - sh_excl[j])]);
+    // Two interleaved streams per lane (products t + 2k kBlock and
+    // t + (2k+1) kBlock), so two gathers are in flight; each stream keeps the
+    // entry of its last product and gallops forward from it.
+    int ja = 0, jb = 0;
+    for (int p = t; p < total; p += 2 * kBlock) {
+      const int pb = p + kBlock;
+      ja = advance_entry(sh_excl, ja, p);
+      const int ca = b_ci[sh_bs[ja] + (p - sh_excl[ja])];
+      int cb = 0;
+      if (pb < total) {
+        jb = advance_entry(sh_excl, max(ja, jb), pb);
+        cb = b_ci[sh_bs[jb] + (pb - sh_excl[jb])];
+      }
+      f(ca);
+      if (pb < total) f(cb);
     }
     __syncthreads();
   }
@@ -1457,24 +1479,37 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
         L.bstart[tid] = bs;
         L.aval[tid] = a;
         __syncthreads();
+        // Software pipeline over groups of kBlock products: the next
+        // group's gathers are issued before this group's lookups and adds.
+        int e = 0;
+        int n_col = 0;
+        float n_b = 0.0f, n_a = 0.0f;
+        if (tid < total) {
+          e = advance_entry(L.pre, e, tid);
+          const int bi = L.bstart[e] + (tid - L.pre[e]);
+          n_col = B.ci[bi];
+          n_b = B.val[bi];
+          n_a = L.aval[e];
+        }
         for (int q0 = 0; q0 < total; q0 += kBlock) {
           const int q = q0 + tid;
+          const int col = n_col;
+          const float bv = n_b, av = n_a;
+          const int qn = q + kBlock;
+          if (qn < total) {
+            // The A entry of product qn: the last e with pre[e] <= qn
+            // (entries without products share their successor's prefix).
+            e = advance_entry(L.pre, e, qn);
+            const int bi = L.bstart[e] + (qn - L.pre[e]);
+            n_col = B.ci[bi];
+            n_b = B.val[bi];
+            n_a = L.aval[e];
+          }
           int pos = -1;
           float prod = 0.0f;
-          if (q < total) {
-            // The A entry of product q: the last e with pre[e] <= q (entries
-            // without products share their successor's prefix).
-            int e = 0;
-#pragma unroll
-            for (int step = kBlock / 2; step > 0; step >>= 1)
-              if (L.pre[e + step] <= q) e += step;
-            const int bi = L.bstart[e] + (q - L.pre[e]);
-            const int col = B.ci[bi];
-            const float bv = B.val[bi];
-            if (col >= clo && col <= chi) {
-              pos = lds_find(L.cols, tn, col);
-              prod = L.aval[e] * bv;  // rounded here, added below
-            }
+          if (q < total && col >= clo && col <= chi) {
+            pos = lds_find(L.cols, tn, col);
+            prod = av * bv;  // rounded here, added below
           }
           bool active = pos >= 0;
           while (__syncthreads_or(active)) {
