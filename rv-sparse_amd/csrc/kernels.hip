@@ -1467,7 +1467,7 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
           a = A.val[ap];
           bs = B.rp[k];
           int be = B.rp[k + 1];
-          if (narrow && be - bs > 32) {
+          if (narrow) {
             bs = gm_lower_bound(B.ci, bs, be, clo);
             be = gm_lower_bound(B.ci, bs, be, chi + 1);
           }
@@ -1479,47 +1479,76 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
         L.bstart[tid] = bs;
         L.aval[tid] = a;
         __syncthreads();
-        // Software pipeline over groups of kBlock products: the next
-        // group's gathers are issued before this group's lookups and adds.
+        // Groups of kItems * kBlock products, kItems per lane (products
+        // q0 + i kBlock + tid), software-pipelined: the next group's gathers
+        // are issued before this group's lookups and adds.
+        constexpr int kItems = 2;
         int e = 0;
-        int n_col = 0;
-        float n_b = 0.0f, n_a = 0.0f;
-        if (tid < total) {
-          e = advance_entry(L.pre, e, tid);
-          const int bi = L.bstart[e] + (tid - L.pre[e]);
-          n_col = B.ci[bi];
-          n_b = B.val[bi];
-          n_a = L.aval[e];
+        int n_col[kItems];
+        float n_b[kItems], n_a[kItems];
+#pragma unroll
+        for (int i = 0; i < kItems; ++i) {
+          const int q = i * kBlock + tid;
+          n_col[i] = -1;
+          n_b[i] = n_a[i] = 0.0f;
+          if (q < total) {
+            e = advance_entry(L.pre, e, q);
+            const int bi = L.bstart[e] + (q - L.pre[e]);
+            n_col[i] = B.ci[bi];
+            n_b[i] = B.val[bi];
+            n_a[i] = L.aval[e];
+          }
         }
-        for (int q0 = 0; q0 < total; q0 += kBlock) {
-          const int q = q0 + tid;
-          const int col = n_col;
-          const float bv = n_b, av = n_a;
-          const int qn = q + kBlock;
-          if (qn < total) {
+        for (int q0 = 0; q0 < total; q0 += kItems * kBlock) {
+          int col[kItems];
+          float bv[kItems], av[kItems];
+#pragma unroll
+          for (int i = 0; i < kItems; ++i) {
+            col[i] = n_col[i];
+            bv[i] = n_b[i];
+            av[i] = n_a[i];
+          }
+#pragma unroll
+          for (int i = 0; i < kItems; ++i) {
             // The A entry of product qn: the last e with pre[e] <= qn
             // (entries without products share their successor's prefix).
-            e = advance_entry(L.pre, e, qn);
-            const int bi = L.bstart[e] + (qn - L.pre[e]);
-            n_col = B.ci[bi];
-            n_b = B.val[bi];
-            n_a = L.aval[e];
-          }
-          int pos = -1;
-          float prod = 0.0f;
-          if (q < total && col >= clo && col <= chi) {
-            pos = lds_find(L.cols, tn, col);
-            prod = av * bv;  // rounded here, added below
-          }
-          bool active = pos >= 0;
-          while (__syncthreads_or(active)) {
-            if (active) atomicMin(&L.tag[pos], tid);
-            __syncthreads();
-            if (active && L.tag[pos] == tid) {
-              L.vals[pos] = L.vals[pos] + prod;
-              L.tag[pos] = INT_MAX;
-              active = false;
+            const int qn = q0 + (kItems + i) * kBlock + tid;
+            n_col[i] = -1;
+            if (qn < total) {
+              e = advance_entry(L.pre, e, qn);
+              const int bi = L.bstart[e] + (qn - L.pre[e]);
+              n_col[i] = B.ci[bi];
+              n_b[i] = B.val[bi];
+              n_a[i] = L.aval[e];
             }
+          }
+          int pos[kItems];
+          float prod[kItems];
+          bool act[kItems];
+#pragma unroll
+          for (int i = 0; i < kItems; ++i) {
+            pos[i] = -1;
+            prod[i] = 0.0f;
+            if (col[i] >= clo && col[i] <= chi) {  // col -1: no product
+              pos[i] = lds_find(L.cols, tn, col[i]);
+              prod[i] = av[i] * bv[i];  // rounded here, added below
+            }
+            act[i] = pos[i] >= 0;
+          }
+          // Ordered adds: per round, the lowest product index among those
+          // still pending on a position wins it.
+          while (__syncthreads_or(act[0] || act[1])) {
+#pragma unroll
+            for (int i = 0; i < kItems; ++i)
+              if (act[i]) atomicMin(&L.tag[pos[i]], i * kBlock + tid);
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < kItems; ++i)
+              if (act[i] && L.tag[pos[i]] == i * kBlock + tid) {
+                L.vals[pos[i]] = L.vals[pos[i]] + prod[i];
+                L.tag[pos[i]] = INT_MAX;
+                act[i] = false;
+              }
           }
         }
         __syncthreads();  // the chunk's metadata is replaced next
