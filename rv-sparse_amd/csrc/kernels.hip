@@ -1100,12 +1100,14 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
   constexpr int kGroups = kBlock / G;
   constexpr int kMapCap = kGroups * kSegMapPerRow;  // map entries staged per segment
   constexpr int kACap = kGroups * kSegAPerRow;      // A entries staged per segment
-  constexpr int kU = 2;
+  constexpr int kU = VAR == 2 ? 3 : 2;
   float* vals = reinterpret_cast<float*>(lds);
   T* mstage = reinterpret_cast<T*>(lds + kGroups * cap);
   float* m_a = reinterpret_cast<float*>(lds + kGroups * cap + (kMapCap * (int)sizeof(T) + 3) / 4);
   int* m_bs = reinterpret_cast<int*>(m_a + kACap);
   int* m_be = m_bs + kACap;
+  __shared__ int s_crp[kGroups + 1], s_arp[kGroups + 1];
+  __shared__ long long s_poff[kGroups + 1];
 
   const int r0 = xcd_block(blockIdx.x, gridDim.x) * kGroups;
   const int nr = min(kGroups, nrows - r0);
@@ -1115,19 +1117,23 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
   const int first = rows[r0];
   const int last = rows[r0 + nr - 1];
   const bool seg = (last - first == nr - 1);
-  const int row = active ? rows[r0 + g] : first;
-  const int c_s = c_rp[row];
-  const int n = active ? c_rp[row + 1] - c_s : 0;
-  const long long mo = poff[row];
-  const int a_s = active ? A.rp[row] : 0;
-  const int a_e = active ? A.rp[row + 1] : 0;
 
+  int c_s = 0, n = 0, a_s = 0, a_e = 0;
+  long long mo = 0;
   int base;
   int s0 = 0, s1 = 0;
   int a_off = 0;          // subtracted from A positions to index staged metadata
   bool a_staged = false;  // uniform
-  const T* rm = map + mo;  // this row's map entries
+  const T* rm;            // this row's map entries
   if (seg) {
+    // Rows first..last: their row pointers of C and A and their map offsets
+    // are staged with coalesced loads, then the C segment's accumulators are
+    // cleared and the segment's map entries staged.
+    for (int i = threadIdx.x; i <= nr; i += kBlock) {
+      s_crp[i] = c_rp[first + i];
+      s_arp[i] = A.rp[first + i];
+      s_poff[i] = poff[first + i];
+    }
     s0 = c_rp[first];
     s1 = c_rp[last + 1];
     const long long m0 = poff[first];
@@ -1150,9 +1156,25 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
       }
     }
     __syncthreads();
+    c_s = s0;
+    mo = m0;
+    if (active) {
+      c_s = s_crp[g];
+      n = s_crp[g + 1] - c_s;
+      mo = s_poff[g];
+      a_s = s_arp[g];
+      a_e = s_arp[g + 1];
+    }
     base = c_s - s0;
-    if (staged) rm = mstage + (mo - m0);
+    rm = staged ? mstage + (mo - m0) : map + mo;
   } else {
+    const int row = active ? rows[r0 + g] : first;
+    c_s = c_rp[row];
+    n = active ? c_rp[row + 1] - c_s : 0;
+    mo = poff[row];
+    a_s = active ? A.rp[row] : 0;
+    a_e = active ? A.rp[row + 1] : 0;
+    rm = map + mo;
     base = g * cap;
     for (int j = l; j < n; j += G) vals[base + j] = 0.0f;
   }
@@ -1176,24 +1198,38 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
     // scales, then their B-row bounds, then the first kU products of each
     // are in flight together (three dependent rounds per chunk instead of
     // one or more per entry); the entries are then applied in order.
-    constexpr int kChunk = 6;
+    constexpr int kChunk = 5;
     if (active) {
       float* rv = vals + base;
       int t0 = 0;
       for (int c0 = a_s; c0 < a_e; c0 += kChunk) {
-        int kk[kChunk], bb[kChunk], ee[kChunk];
+        int bb[kChunk], ee[kChunk];
         float aa[kChunk], vv[kChunk][kU];
+        // The chunk's A entries are split over the group's lanes (lane l
+        // loads entries l, l + G, ...) and shared by shuffles, so no entry
+        // is loaded twice.
+        constexpr int kPer = (kChunk + G - 1) / G;
+        int kl[kPer], bl[kPer], el[kPer];
+        float al[kPer];
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-          const bool h = c0 + u < a_e;
-          kk[u] = h ? A.ci[c0 + u] : 0;
-          aa[u] = h ? A.val[c0 + u] : 0.0f;
+        for (int u = 0; u < kPer; ++u) {
+          const int e = c0 + l + u * G;
+          const bool h = l + u * G < kChunk && e < a_e;
+          kl[u] = h ? A.ci[e] : 0;
+          al[u] = h ? A.val[e] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+          const int e = c0 + l + u * G;
+          const bool h = l + u * G < kChunk && e < a_e;
+          bl[u] = h ? B.rp[kl[u]] : 0;
+          el[u] = h ? B.rp[kl[u] + 1] : 0;
         }
 #pragma unroll
         for (int u = 0; u < kChunk; ++u) {
-          const bool h = c0 + u < a_e;
-          bb[u] = h ? B.rp[kk[u]] : 0;
-          ee[u] = h ? B.rp[kk[u] + 1] : 0;
+          aa[u] = __shfl(al[u / G], u % G, G);
+          bb[u] = __shfl(bl[u / G], u % G, G);
+          ee[u] = __shfl(el[u / G], u % G, G);
         }
 #pragma unroll
         for (int u = 0; u < kChunk; ++u)
