@@ -422,7 +422,9 @@ rvsp_status_t plan_map(rvsp_spgemm_descr* d, hipStream_t s) {
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
   const size_t budget = std::min<size_t>((size_t)8 << 30, free_b / 4);
-  if (bytes > budget || ensure(d->map, bytes) != RVSP_SUCCESS) {
+  // 64 bytes of slack: the map kernels stage a segment's entries with
+  // aligned 16-byte loads that may read up to 15 bytes past its end.
+  if (bytes > budget || ensure(d->map, bytes + 64) != RVSP_SUCCESS) {
     // Over budget: these bins keep searching C's columns instead.
     for (int b = 2; b < p.table.rule.nbins; ++b)
       if ((mask >> b) & 1u) {

@@ -1103,7 +1103,7 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
   constexpr int kU = VAR == 2 ? 3 : 2;
   float* vals = reinterpret_cast<float*>(lds);
   T* mstage = reinterpret_cast<T*>(lds + kGroups * cap);
-  float* m_a = reinterpret_cast<float*>(lds + kGroups * cap + (kMapCap * (int)sizeof(T) + 3) / 4);
+  float* m_a = reinterpret_cast<float*>(lds + kGroups * cap + map_stage_bytes(kGroups) / 4);
   int* m_bs = reinterpret_cast<int*>(m_a + kACap);
   int* m_be = m_bs + kACap;
   __shared__ int s_crp[kGroups + 1], s_arp[kGroups + 1];
@@ -1144,8 +1144,24 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
       a_staged = A.rp[last + 1] - a_off <= kACap;
     }
     for (int i = threadIdx.x; i < s1 - s0; i += kBlock) vals[i] = 0.0f;
-    if (staged)
-      for (int i = threadIdx.x; i < (int)(m1 - m0); i += kBlock) mstage[i] = map[m0 + i];
+    int mshift = 0;  // bytes from the staged copy's start to entry m0
+    if (staged) {
+      // Aligned 16-byte loads over the segment's bytes (the map buffer has
+      // slack past its end for the last one).
+      const long long b0 = m0 * (long long)sizeof(T);
+      const long long ab0 = b0 & ~15LL;
+      const int n16 = (int)((m1 * (long long)sizeof(T) - ab0 + 15) >> 4);
+      const uint4* src = reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(map) + ab0);
+      unsigned* dst = reinterpret_cast<unsigned*>(mstage);  // 4-byte aligned in LDS
+      for (int i = threadIdx.x; i < n16; i += kBlock) {
+        const uint4 v = src[i];
+        dst[4 * i] = v.x;
+        dst[4 * i + 1] = v.y;
+        dst[4 * i + 2] = v.z;
+        dst[4 * i + 3] = v.w;
+      }
+      mshift = (int)(b0 - ab0);
+    }
     if (a_staged) {
       const int na = A.rp[last + 1] - a_off;
       for (int i = threadIdx.x; i < na; i += kBlock) {
@@ -1166,7 +1182,9 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
       a_e = s_arp[g + 1];
     }
     base = c_s - s0;
-    rm = staged ? mstage + (mo - m0) : map + mo;
+    rm = staged ? reinterpret_cast<const T*>(reinterpret_cast<const char*>(mstage) + mshift) +
+                      (mo - m0)
+                : map + mo;
   } else {
     const int row = active ? rows[r0 + g] : first;
     c_s = c_rp[row];

@@ -169,5 +169,8 @@ constexpr int kSegAPerRow = 6;
 
 // Map entries per row the MAP kernel's segment mode stages in LDS.
 constexpr int kSegMapPerRow = 32;
+// LDS bytes of a segment's staged map entries (2-byte entries at most, plus
+// room for the 16-byte alignment of the staging loads).
+constexpr int map_stage_bytes(int groups) { return (groups * kSegMapPerRow * 2 + 32 + 15) / 16 * 16; }
 
 }  // namespace rvsp
