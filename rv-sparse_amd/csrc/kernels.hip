@@ -504,8 +504,10 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_row(
                        });
     // block_for_each_col ends with a barrier, which also orders the global
     // atomics above before the reads below (workgroup scope).
+    // Every thread scans a run of `per` consecutive words; an odd run length
+    // puts the 64 lanes' words of one step in distinct banks.
     const int nw = w1 - w0;
-    const int per = (nw + kBlock - 1) / kBlock;
+    const int per = ((nw + kBlock - 1) / kBlock) | 1;
     const int my0 = w0 + min(nw, (int)threadIdx.x * per);
     const int my1 = w0 + min(nw, ((int)threadIdx.x + 1) * per);
     int c = 0;
