@@ -52,6 +52,13 @@ rvsp_status_t rvsp_spgemm_get_stream(rvsp_spgemm_descr_t descr,
  * mode always waits. */
 rvsp_status_t rvsp_spgemm_set_async(rvsp_spgemm_descr_t descr, int async);
 
+/* on != 0 (default): from the first numeric-only compute on (C.col_idx is
+ * the array filled before), the descriptor's searching numeric bins switch to
+ * the position map (RVSP_KIND_MAP), built once on that compute; that compute
+ * synchronises. Tuned plans keep their measured shapes. Output bits are the
+ * same either way. */
+rvsp_status_t rvsp_spgemm_set_position_map(rvsp_spgemm_descr_t descr, int on);
+
 /* Phase ids used by the plan queries below. */
 typedef enum
 {

@@ -109,6 +109,7 @@ struct BatchState {
   std::vector<const void*> graph_key;
   bool graph_failed = false;
   int graph_launches = 0;
+  int repeats = 0;  // consecutive numeric-only computes with the same arrays
 
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -498,6 +499,7 @@ rvsp_status_t rvsp_spgemm_batch_compute(rvsp_spgemm_descr_t descr, int32_t count
     RVSP_HIP_TRY(bs.t_struct.launch(s));
     bs.struct_key = sk;
     for (BatchGroup& g : bs.groups) rvsp_spgemm_invalidate_structure(g.inner);
+    bs.out_filled = false;
   }
   std::vector<const void*> vk = value_key(count, A, B);
   if (vk != bs.vals_key) {
@@ -534,7 +536,10 @@ rvsp_status_t rvsp_spgemm_batch_compute(rvsp_spgemm_descr_t descr, int32_t count
   key.insert(key.end(), ok.begin(), ok.end());
   const bool repeat = bs.out_filled && key == bs.last_key;
   bs.last_key = key;
-  if (repeat && d->batch_graph && !bs.graph_failed) {
+  bs.repeats = repeat ? bs.repeats + 1 : 0;
+  // The first numeric-only compute runs eagerly (the groups may switch to
+  // their position maps, which synchronises); the second is captured.
+  if (repeat && bs.repeats >= 2 && d->batch_graph && !bs.graph_failed) {
     if (!bs.exec) {
       // Capture one numeric-only compute; later ones replay it.
       RVSP_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));

@@ -133,6 +133,7 @@ void drop_structure(rvsp_spgemm_descr* d) {
   d->map_valid = false;
   d->map_bins = 0;
   d->map_width = 0;
+  d->map_promoted = false;
   d->estimated = 0;
   d->c_nnz = 0;
 }
@@ -454,6 +455,30 @@ rvsp_status_t ensure_map_built(rvsp_spgemm_descr* d, const DevCsr& A, const DevC
   return RVSP_SUCCESS;
 }
 
+rvsp_status_t promote_to_map(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
+                             const int* c_ci, hipStream_t s) {
+  d->map_promoted = true;
+  PhaseTable& t = d->numeric.table;
+  bool any = false;
+  for (int b = 2; b < t.rule.nbins; ++b) {
+    if (bin_rows(d->numeric, b) == 0 || t.spec[b].kind != KIND_SEARCH) continue;
+    BinChoice c;
+    c.kind = KIND_MAP;
+    c.group = t.spec[b].group;
+    c.cap = t.spec[b].cap;
+    c.variant = 2;
+    PhaseTable trial = t;  // a refused choice would turn the bin GLOBAL
+    if (apply_choice(trial, PHASE_NUMERIC, b, c, t.rule.bound[b])) {
+      t.spec[b] = trial.spec[b];
+      any = true;
+    }
+  }
+  if (!any) return RVSP_SUCCESS;
+  RVSP_TRY(plan_map(d, s));  // bins over the memory budget go back to SEARCH
+  if (d->map_width) RVSP_TRY(ensure_map_built(d, A, B, c_ci, s));
+  return RVSP_SUCCESS;
+}
+
 }  // namespace rvsp
 
 extern "C" {
@@ -592,6 +617,11 @@ rvsp_status_t rvsp_spgemm_compute(rvsp_spgemm_descr_t descr,
     if (d->filled_into != C->col_idx) {
       RVSP_TRY(run_fill(d, dA, dB, C->col_idx, s));
       d->filled_into = C->col_idx;
+    } else if (d->position_map && !d->map_promoted && d->algo != RVSP_SPGEMM_ALGO_HIP_TUNED) {
+      // A numeric-only compute: the structure is being reused, so its
+      // searching bins switch to the position map from here on (tuned plans
+      // keep the shapes their measurements chose).
+      RVSP_TRY(promote_to_map(d, dA, dB, C->col_idx, s));
     }
     RVSP_TRY(run_numeric(d, dA, dB, C->col_idx, (float*)C->values, s));
     if (!d->async) RVSP_HIP_TRY(hipStreamSynchronize(s));
@@ -602,6 +632,8 @@ rvsp_status_t rvsp_spgemm_compute(rvsp_spgemm_descr_t descr,
       RVSP_TRY(run_fill(d, dA, dB, (int*)d->c_ci_dev.p, s));
       d->dev_cols_valid = true;
       d->filled_into = NULL;
+    } else if (d->position_map && !d->map_promoted && d->algo != RVSP_SPGEMM_ALGO_HIP_TUNED) {
+      RVSP_TRY(promote_to_map(d, dA, dB, (const int*)d->c_ci_dev.p, s));
     }
     RVSP_TRY(run_numeric(d, dA, dB, (const int*)d->c_ci_dev.p, (float*)d->c_val_dev.p, s));
     if (d->filled_into != C->col_idx && d->c_nnz > 0)
@@ -812,6 +844,12 @@ rvsp_status_t rvsp_spgemm_get_stream(rvsp_spgemm_descr_t descr, void** hip_strea
 rvsp_status_t rvsp_spgemm_set_async(rvsp_spgemm_descr_t descr, int async) {
   if (descr == NULL) return RVSP_ERROR_NULL_POINTER;
   descr->async = async != 0;
+  return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_spgemm_set_position_map(rvsp_spgemm_descr_t descr, int on) {
+  if (descr == NULL) return RVSP_ERROR_NULL_POINTER;
+  descr->position_map = on != 0;
   return RVSP_SUCCESS;
 }
 

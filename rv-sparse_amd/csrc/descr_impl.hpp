@@ -83,6 +83,8 @@ struct rvsp_spgemm_descr {
   int map_width = 0;        // bytes per entry; 0 = no map planned
   unsigned map_bins = 0;    // numeric bins whose rows have map entries
   bool map_valid = false;   // entries built for the current structure
+  bool map_promoted = false;  // searching bins already moved to the map
+  bool position_map = true;   // promote on the first numeric-only compute
 
   // Batched products (batch.cpp): stacked operands and one inner
   // descriptor per group of pairs; settings apply at the next estimation.
@@ -111,6 +113,11 @@ rvsp_status_t plan_map(rvsp_spgemm_descr* d, hipStream_t s);
 // Builds the map entries if they are not valid for the current structure.
 rvsp_status_t ensure_map_built(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
                                const int* c_ci, hipStream_t s);
+// Moves the non-empty searching numeric bins to the position map (chunked
+// variant, same lanes and capacity) and builds the map; bins over the memory
+// budget keep searching. Synchronous.
+rvsp_status_t promote_to_map(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
+                             const int* c_ci, hipStream_t s);
 
 void destroy_graph(rvsp_spgemm_descr* d);
 void destroy_batch(rvsp_spgemm_descr* d);

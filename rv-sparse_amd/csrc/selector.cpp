@@ -176,6 +176,7 @@ rvsp_status_t autotune_plan(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr&
   RVSP_HIP_TRY(hipEventCreate(&e1));
   PhasePlan* plans[] = {&d->count, &d->fill, &d->numeric};
   rvsp_status_t status = RVSP_SUCCESS;
+  d->map_promoted = true;  // the measured choice stands for later computes
   // Give every searching numeric bin a position map when the memory budget
   // allows, so searching and the map can be timed against each other.
   {

@@ -118,6 +118,7 @@ PROTOTYPES = {
     "rvsp_spgemm_set_stream": (C.c_int, [P, P]),
     "rvsp_spgemm_get_stream": (C.c_int, [P, C.POINTER(P)]),
     "rvsp_spgemm_set_async": (C.c_int, [P, C.c_int]),
+    "rvsp_spgemm_set_position_map": (C.c_int, [P, C.c_int]),
     "rvsp_spgemm_get_plan": (C.c_int, [P, C.POINTER(BinInfo), I32, C.POINTER(I32)]),
     "rvsp_spgemm_set_bin_choices": (C.c_int, [P, C.POINTER(BinChoice), I32]),
     "rvsp_spgemm_autotune": (C.c_int, [P, PM, PM, PM, P, I32]),
