@@ -106,18 +106,23 @@ __device__ __forceinline__ void block_for_each_col(int a_s, int a_e,
     // Two interleaved streams per lane (products t + 2k kBlock and
     // t + (2k+1) kBlock), so two gathers are in flight; each stream keeps the
     // entry of its last product and gallops forward from it.
+    // The loop is uniform over the block (f sees every lane, with a flag for
+    // lanes past the end), so f may use cross-lane operations.
     int ja = 0, jb = 0;
-    for (int p = t; p < total; p += 2 * kBlock) {
+    for (int p0 = 0; p0 < total; p0 += 2 * kBlock) {
+      const int p = p0 + t;
       const int pb = p + kBlock;
-      ja = advance_entry(sh_excl, ja, p);
-      const int ca = b_ci[sh_bs[ja] + (p - sh_excl[ja])];
-      int cb = 0;
+      int ca = -1, cb = -1;
+      if (p < total) {
+        ja = advance_entry(sh_excl, ja, p);
+        ca = b_ci[sh_bs[ja] + (p - sh_excl[ja])];
+      }
       if (pb < total) {
         jb = advance_entry(sh_excl, max(ja, jb), pb);
         cb = b_ci[sh_bs[jb] + (pb - sh_excl[jb])];
       }
-      f(ca);
-      if (pb < total) f(cb);
+      f(ca, p < total);
+      f(cb, pb < total);
     }
     __syncthreads();
   }
@@ -495,39 +500,57 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_row(
       for (int w = threadIdx.x; w < w1; w += kBlock) bits[w] = 0u;
       __syncthreads();
     }
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x >> 6;
     block_for_each_col(A.rp[row], A.rp[row + 1], A.ci, B.rp, B.ci, sh_excl,
-                       sh_bs, tmp, [&](int col) {
-                         const int d = col - base_col;
-                         __hip_atomic_fetch_or(&bits[d >> 5], 1u << (d & 31),
-                                               __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                       sh_bs, tmp, [&](int col, bool valid) {
+                         if (valid) {
+                           const int d = col - base_col;
+                           __hip_atomic_fetch_or(&bits[d >> 5], 1u << (d & 31),
+                                                 __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+                         }
                        });
     // block_for_each_col ends with a barrier, which also orders the global
     // atomics above before the reads below (workgroup scope).
-    // Every thread scans a run of `per` consecutive words; an odd run length
-    // puts the 64 lanes' words of one step in distinct banks.
+    // The span's words are split into one quarter per wave; a wave reads its
+    // quarter 64 consecutive words at a time (no bank conflicts).
     const int nw = w1 - w0;
-    const int per = ((nw + kBlock - 1) / kBlock) | 1;
-    const int my0 = w0 + min(nw, (int)threadIdx.x * per);
-    const int my1 = w0 + min(nw, ((int)threadIdx.x + 1) * per);
+    const int q0 = w0 + (int)((long long)nw * wid / (kBlock / kWave));
+    const int q1 = w0 + (int)((long long)nw * (wid + 1) / (kBlock / kWave));
     int c = 0;
-    for (int w = my0; w < my1; ++w) c += __popc(bits[w]);
-    int total;
-    int off = block_excl_scan(c, tmp, &total);
+    for (int w = q0 + lane; w < q1; w += kWave) {
+      c += __popc(bits[w]);
+      if (GLOBAL_BM && !FILL) bits[w] = 0u;
+    }
+    c = group_sum<kWave>(c);
+    if (lane == 0) tmp[wid] = c;
+    __syncthreads();
+    int total = 0, base = 0;
+#pragma unroll
+    for (int v = 0; v < kBlock / kWave; ++v) {
+      total += tmp[v];
+      base += v < wid ? tmp[v] : 0;
+    }
     if (!FILL) {
       if (threadIdx.x == 0) cnt[row] = total;
-      if (GLOBAL_BM)
-        for (int w = my0; w < my1; ++w) bits[w] = 0u;
     } else {
+      // Ascending output: a wave's 64 words at a time, each lane's columns
+      // after those of the lanes below it (a wave prefix sum of popcounts).
       int* out = c_ci + c_rp[row];
-      for (int w = my0; w < my1; ++w) {
-        unsigned m = bits[w];
-        if (GLOBAL_BM) bits[w] = 0u;
+      for (int wb = q0; wb < q1; wb += kWave) {
+        const int w = wb + lane;
+        unsigned m = w < q1 ? bits[w] : 0u;
+        if (GLOBAL_BM && w < q1) bits[w] = 0u;
+        const int pc = __popc(m);
+        const int incl = group_incl_scan<kWave>(pc);
+        int off = base + incl - pc;
         while (m) {
           const int b = __ffs(m) - 1;
           out[off++] = base_col + w * 32 + b;
           m &= m - 1;
         }
+        base += __shfl(incl, kWave - 1, kWave);
       }
     }
     __syncthreads();
