@@ -134,6 +134,8 @@ void drop_structure(rvsp_spgemm_descr* d) {
   d->map_bins = 0;
   d->map_width = 0;
   d->map_promoted = false;
+  d->keep_ok = false;
+  d->keep_bins = 0;
   d->estimated = 0;
   d->c_nnz = 0;
 }
@@ -283,6 +285,36 @@ rvsp_status_t select_phase(rvsp_spgemm_descr* d, PhasePlan& p, int phase, const 
   return RVSP_SUCCESS;
 }
 
+// Decides whether the count phase's LDS-bitmap rows keep their sorted
+// columns for the fill phase (one bitmap pass instead of two): offsets from
+// the rows' column bounds min(Op_i, span_i), storage within
+// min(16 GiB, free / 4); otherwise those rows are filled as usual.
+rvsp_status_t plan_keep(rvsp_spgemm_descr* d, int M, hipStream_t s) {
+  d->keep_ok = false;
+  d->keep_bins = 0;
+  unsigned mask = 0;
+  for (int b = 2; b < d->count.table.rule.nbins; ++b)
+    if (d->count.table.spec[b].kind == KIND_BITMAP && bin_rows(d->count, b) > 0) mask |= 1u << b;
+  if (mask == 0) return RVSP_SUCCESS;
+  RVSP_TRY(ensure(d->koff, ((size_t)M + 1) * 8));
+  RVSP_TRY(ensure(d->keep_w, (size_t)M * 8));
+  RVSP_HIP_TRY(launch_map_offsets(M, (const unsigned char*)d->bin_of.p, mask, row_stats(d),
+                                  (long long*)d->keep_w.p, (long long*)d->koff.p,
+                                  (long long*)d->scan_tiles.p, (long long*)d->total.p, s,
+                                  true));
+  long long total = 0;
+  RVSP_HIP_TRY(hipMemcpyAsync(&total, d->total.p, 8, hipMemcpyDeviceToHost, s));
+  RVSP_HIP_TRY(hipStreamSynchronize(s));
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  const size_t bytes = (size_t)total * 4;
+  const size_t budget = std::min<size_t>((size_t)16 << 30, free_b / 4);
+  if (bytes > budget || ensure(d->keep, bytes) != RVSP_SUCCESS) return RVSP_SUCCESS;
+  d->keep_bins = mask;
+  d->keep_ok = true;
+  return RVSP_SUCCESS;
+}
+
 // The symbolic phase on the device: analysis, binning, exact count, scan,
 // then the fill and numeric plans. On success *total holds nnz(C).
 rvsp_status_t estimate(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
@@ -322,12 +354,16 @@ rvsp_status_t estimate(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
   RVSP_TRY(fetch_offsets(d, cp, 1, 0, s));
   RVSP_TRY(select_phase(d, d->count, PHASE_COUNT, A, nullptr, s));
   RVSP_TRY(ensure_slots(d, d->count, B.cols, false, s));
+  RVSP_TRY(plan_keep(d, M, s));
   for (int b = 2; b < d->count.table.rule.nbins; ++b) {
     const int n = bin_rows(d->count, b);
     if (n == 0) continue;
+    const bool kb = d->keep_ok && ((d->keep_bins >> b) & 1u);
     RVSP_HIP_TRY(launch_count_bin(d->count.table.spec[b],
                                   (const int*)d->count.rows.p + d->count.offsets[b],
-                                  n, A, B, st, (int*)d->cnt.p, d->gs_bits, s));
+                                  n, A, B, st, (int*)d->cnt.p, d->gs_bits, s,
+                                  kb ? (int*)d->keep.p : nullptr,
+                                  kb ? (const long long*)d->koff.p : nullptr));
   }
   RVSP_HIP_TRY(launch_scan((const int*)d->cnt.p, M, (int*)d->c_rp.p,
                            (long long*)d->total.p, (long long*)d->scan_tiles.p, s));
@@ -339,7 +375,8 @@ rvsp_status_t estimate(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
 
   // Fill and numeric plans from the exact per-row nnz.
   RVSP_HIP_TRY(launch_classify_nnz(A, st, (const int*)d->c_rp.p, d->fill.table.rule,
-                                   (unsigned char*)d->bin_of.p, s));
+                                   (unsigned char*)d->bin_of.p, s,
+                                   d->keep_ok ? (const long long*)d->koff.p : nullptr));
   RVSP_TRY(partition_phase(d, d->fill, M, 1, s));
   RVSP_HIP_TRY(launch_classify_nnz(A, st, (const int*)d->c_rp.p,
                                    d->numeric.table.rule,
@@ -362,6 +399,14 @@ namespace rvsp {
 rvsp_status_t run_fill(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
                        int* c_ci, hipStream_t s) {
   const RowStats st = row_stats(d);
+  if (d->keep_ok)
+    for (int b = 2; b < d->count.table.rule.nbins; ++b) {
+      if (!((d->keep_bins >> b) & 1u)) continue;
+      RVSP_HIP_TRY(launch_copy_kept((const int*)d->count.rows.p + d->count.offsets[b],
+                                    bin_rows(d->count, b), (const int*)d->keep.p,
+                                    (const long long*)d->koff.p, (const int*)d->c_rp.p,
+                                    c_ci, s));
+    }
   for (int b = 1; b < d->fill.table.rule.nbins; ++b) {
     const int n = bin_rows(d->fill, b);
     if (n == 0) continue;
@@ -506,7 +551,8 @@ void rvsp_spgemm_descr_destroy(rvsp_spgemm_descr_t descr) {
                     &d->tile_hist, &d->tile_base, &d->bin_offsets, &d->count.rows,
                     &d->fill.rows, &d->numeric.rows, &d->c_ci_dev, &d->c_val_dev,
                     &d->gbits, &d->gdense, &d->learn_feat, &d->bin_stats,
-                    &d->map, &d->poff, &d->map_weights};
+                    &d->map, &d->poff, &d->map_weights, &d->keep, &d->koff,
+                    &d->keep_w};
   for (DevBuf* b : bufs) release(*b);
   destroy_graph(d);
   destroy_batch(d);

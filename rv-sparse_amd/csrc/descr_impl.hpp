@@ -92,6 +92,12 @@ struct rvsp_spgemm_descr {
   bool batch_graph = true;
   long long batch_limit = 0;  // products per group; 0 = INT32_MAX
 
+  // Sorted columns of the count phase's LDS-bitmap rows, kept for the fill
+  // phase (offsets koff: exclusive scan of min(Op_i, span_i) over them).
+  rvsp::DevBuf keep, koff, keep_w;
+  unsigned keep_bins = 0;  // count bins whose rows keep their columns
+  bool keep_ok = false;
+
   // Captured numeric-only graph.
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;

@@ -244,13 +244,13 @@ __global__ __launch_bounds__(kBlock) void k_classify_count(
 
 __global__ __launch_bounds__(kBlock) void k_classify_nnz(
     int M, RowStats st, const int* __restrict__ c_rp, BinRule r,
-    unsigned char* __restrict__ bin_of) {
+    unsigned char* __restrict__ bin_of, const long long* __restrict__ kept) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
   const int n = c_rp[i + 1] - c_rp[i];
   int b;
-  if (n == 0)
-    b = 0;
+  if (n == 0 || (kept && kept[i + 1] > kept[i]))
+    b = 0;  // nothing to do, or columns kept by the count phase
   else if ((long long)st.maxb[i] == st.ops[i])
     b = 1;
   else
@@ -268,10 +268,10 @@ hipError_t launch_classify_count(const DevCsr& A, RowStats st, const BinRule& r,
 
 hipError_t launch_classify_nnz(const DevCsr& A, RowStats st, const int* c_rp,
                                const BinRule& r, unsigned char* bin_of,
-                               hipStream_t s) {
+                               hipStream_t s, const long long* kept) {
   const int grid = (A.rows + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_classify_nnz, dim3(grid), dim3(kBlock), 0, s, A.rows, st,
-                     c_rp, r, bin_of);
+                     c_rp, r, bin_of, kept);
   return hipGetLastError();
 }
 
@@ -476,16 +476,23 @@ __global__ __launch_bounds__(kBlock) void k_count_hash(
   if (l == 0) cnt[row] = mine;
 }
 
+// k_bitmap_row modes: count only; write the sorted columns to C; count and
+// also keep the sorted columns at koff[row] of a side buffer, so the fill
+// phase copies them instead of building the bitmap again.
+constexpr int kBitmapCount = 0, kBitmapFill = 1, kBitmapCountKeep = 2;
+
 // One workgroup per row. The row's columns lie in [lo, hi]; a bitmap over that
 // span lives in LDS (GLOBAL_BM == false, span <= 32 * words) or in this
 // workgroup's slot of global scratch (GLOBAL_BM == true, absolute column
 // bits, words cleared again after use). FILL == false counts set bits;
 // FILL == true writes the set columns in ascending order to C.
-template <bool GLOBAL_BM, bool FILL>
+template <bool GLOBAL_BM, int MODE>
 __global__ __launch_bounds__(kBlock) void k_bitmap_row(
     const int* __restrict__ rows, int nrows, DevCsr A, DevCsr B, RowStats st,
     int* __restrict__ cnt, const int* __restrict__ c_rp, int* __restrict__ c_ci,
-    unsigned* __restrict__ gbits, size_t slot_words) {
+    unsigned* __restrict__ gbits, size_t slot_words, int* __restrict__ keep,
+    const long long* __restrict__ koff) {
+  constexpr bool FILL = MODE == kBitmapFill;
   extern __shared__ unsigned lbits[];
   __shared__ int sh_excl[kBlock], sh_bs[kBlock], tmp[kBlock / kWave];
   unsigned* bits = GLOBAL_BM ? gbits + (size_t)blockIdx.x * slot_words : lbits;
@@ -532,12 +539,11 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_row(
       total += tmp[v];
       base += v < wid ? tmp[v] : 0;
     }
-    if (!FILL) {
-      if (threadIdx.x == 0) cnt[row] = total;
-    } else {
+    if (MODE != kBitmapFill && threadIdx.x == 0) cnt[row] = total;
+    if (MODE != kBitmapCount) {
       // Ascending output: a wave's 64 words at a time, each lane's columns
       // after those of the lanes below it (a wave prefix sum of popcounts).
-      int* out = c_ci + c_rp[row];
+      int* out = FILL ? c_ci + c_rp[row] : keep + koff[row];
       for (int wb = q0; wb < q1; wb += kWave) {
         const int w = wb + lane;
         unsigned m = w < q1 ? bits[w] : 0u;
@@ -574,7 +580,8 @@ static int row_grid(int nrows) { return nrows < 65535 ? nrows : 65535; }
 
 hipError_t launch_count_bin(const BinSpec& b, const int* rows, int nrows,
                             const DevCsr& A, const DevCsr& B, RowStats st,
-                            int* cnt, const GlobalSlots& gs, hipStream_t s) {
+                            int* cnt, const GlobalSlots& gs, hipStream_t s, int* keep,
+                            const long long* koff) {
   if (nrows <= 0) return hipSuccess;
   switch (b.kind) {
     case KIND_HASH:
@@ -589,16 +596,23 @@ hipError_t launch_count_bin(const BinSpec& b, const int* rows, int nrows,
         default: return hipErrorInvalidValue;
       }
     case KIND_BITMAP:
-      hipLaunchKernelGGL((k_bitmap_row<false, false>), dim3(row_grid(nrows)),
-                         dim3(kBlock), b.lds_bytes, s, rows, nrows, A, B, st,
-                         cnt, nullptr, nullptr, nullptr, (size_t)0);
+      if (keep) {
+        hipLaunchKernelGGL((k_bitmap_row<false, kBitmapCountKeep>), dim3(row_grid(nrows)),
+                           dim3(kBlock), b.lds_bytes, s, rows, nrows, A, B, st, cnt,
+                           nullptr, nullptr, nullptr, (size_t)0, keep, koff);
+      } else {
+        hipLaunchKernelGGL((k_bitmap_row<false, kBitmapCount>), dim3(row_grid(nrows)),
+                           dim3(kBlock), b.lds_bytes, s, rows, nrows, A, B, st, cnt,
+                           nullptr, nullptr, nullptr, (size_t)0, nullptr, nullptr);
+      }
       return hipGetLastError();
     case KIND_GLOBAL:
       if (!gs.base || gs.nslots <= 0) return hipErrorInvalidValue;
-      hipLaunchKernelGGL((k_bitmap_row<true, false>),
+      hipLaunchKernelGGL((k_bitmap_row<true, kBitmapCount>),
                          dim3(min(nrows, gs.nslots)), dim3(kBlock), 0, s, rows,
                          nrows, A, B, st, cnt, nullptr, nullptr,
-                         (unsigned*)gs.base, gs.slot_bytes / sizeof(unsigned));
+                         (unsigned*)gs.base, gs.slot_bytes / sizeof(unsigned), nullptr,
+                         nullptr);
       return hipGetLastError();
     default:
       return hipErrorInvalidValue;
@@ -787,16 +801,16 @@ hipError_t launch_fill_bin(const BinSpec& b, const int* rows, int nrows,
         default: return hipErrorInvalidValue;
       }
     case KIND_BITMAP:
-      hipLaunchKernelGGL((k_bitmap_row<false, true>), dim3(row_grid(nrows)),
+      hipLaunchKernelGGL((k_bitmap_row<false, kBitmapFill>), dim3(row_grid(nrows)),
                          dim3(kBlock), b.lds_bytes, s, rows, nrows, A, B, st,
-                         nullptr, c_rp, c_ci, nullptr, (size_t)0);
+                         nullptr, c_rp, c_ci, nullptr, (size_t)0, nullptr, nullptr);
       return hipGetLastError();
     case KIND_GLOBAL:
       if (!gs.base || gs.nslots <= 0) return hipErrorInvalidValue;
-      hipLaunchKernelGGL((k_bitmap_row<true, true>), dim3(min(nrows, gs.nslots)),
+      hipLaunchKernelGGL((k_bitmap_row<true, kBitmapFill>), dim3(min(nrows, gs.nslots)),
                          dim3(kBlock), 0, s, rows, nrows, A, B, st, nullptr,
                          c_rp, c_ci, (unsigned*)gs.base,
-                         gs.slot_bytes / sizeof(unsigned));
+                         gs.slot_bytes / sizeof(unsigned), nullptr, nullptr);
       return hipGetLastError();
     default:
       return hipErrorInvalidValue;
@@ -1012,9 +1026,16 @@ __global__ __launch_bounds__(kBlock) void k_numeric_search(
 // w[i] = Op_i for rows whose numeric bin is in `bin_mask`, else 0.
 __global__ __launch_bounds__(kBlock) void k_map_weights(int M, const unsigned char* __restrict__ bin_of,
                                                         unsigned bin_mask, RowStats st,
-                                                        long long* __restrict__ w) {
+                                                        long long* __restrict__ w,
+                                                        bool span_bound) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < M) w[i] = ((bin_mask >> bin_of[i]) & 1u) ? st.ops[i] : 0;
+  if (i >= M) return;
+  long long v = 0;
+  if ((bin_mask >> bin_of[i]) & 1u) {
+    v = st.ops[i];
+    if (span_bound && v > 0) v = min(v, (long long)(st.hi[i] - st.lo[i] + 1));
+  }
+  w[i] = v;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scan64_reduce(const long long* __restrict__ in, int n,
@@ -1057,11 +1078,11 @@ __global__ __launch_bounds__(kBlock) void k_scan64_apply(const long long* __rest
 
 hipError_t launch_map_offsets(int M, const unsigned char* bin_of, unsigned bin_mask, RowStats st,
                               long long* weights, long long* poff, long long* tile_sums,
-                              long long* total, hipStream_t s) {
+                              long long* total, hipStream_t s, bool span_bound) {
   if (M <= 0) return hipSuccess;
   const int ntiles = (M + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(k_map_weights, dim3((M + kBlock - 1) / kBlock), dim3(kBlock), 0, s, M,
-                     bin_of, bin_mask, st, weights);
+                     bin_of, bin_mask, st, weights, span_bound);
   hipLaunchKernelGGL(k_scan64_reduce, dim3(ntiles), dim3(kBlock), 0, s, weights, M, tile_sums);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, s, tile_sums, ntiles, total);
   hipLaunchKernelGGL(k_scan64_apply, dim3(ntiles), dim3(kBlock), 0, s, weights, M, tile_sums, poff);
@@ -1698,6 +1719,31 @@ hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
   }
 }
 
+// Fill of rows whose sorted columns the count phase kept: one workgroup per
+// row copies them to C.
+__global__ __launch_bounds__(kBlock) void k_copy_kept(const int* __restrict__ rows, int nrows,
+                                                      const int* __restrict__ keep,
+                                                      const long long* __restrict__ koff,
+                                                      const int* __restrict__ c_rp,
+                                                      int* __restrict__ c_ci) {
+  for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
+    const int row = rows[r];
+    const int c_s = c_rp[row];
+    const int n = c_rp[row + 1] - c_s;
+    const int* src = keep + koff[row];
+    for (int j = threadIdx.x; j < n; j += kBlock) c_ci[c_s + j] = src[j];
+  }
+}
+
+hipError_t launch_copy_kept(const int* rows, int nrows, const int* keep,
+                            const long long* koff, const int* c_rp, int* c_ci,
+                            hipStream_t s) {
+  if (nrows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_kept, dim3(row_grid(nrows)), dim3(kBlock), 0, s, rows, nrows,
+                     keep, koff, c_rp, c_ci);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 
 template <typename K>
@@ -1722,7 +1768,9 @@ hipError_t init_kernel_attributes(int lds_bytes) {
   RVSP_ALLOW(k_numeric_search<4>) RVSP_ALLOW(k_numeric_search<8>)
   RVSP_ALLOW(k_numeric_search<16>) RVSP_ALLOW(k_numeric_search<32>)
   RVSP_ALLOW(k_numeric_search<64>)
-  RVSP_ALLOW((k_bitmap_row<false, false>)) RVSP_ALLOW((k_bitmap_row<false, true>))
+  RVSP_ALLOW((k_bitmap_row<false, kBitmapCount>))
+  RVSP_ALLOW((k_bitmap_row<false, kBitmapFill>))
+  RVSP_ALLOW((k_bitmap_row<false, kBitmapCountKeep>))
   RVSP_ALLOW(k_numeric_rowblock<false>) RVSP_ALLOW(k_numeric_tiled)
 #define RVSP_ALLOW_MAP(G)                                                     \
   RVSP_ALLOW((k_numeric_map<G, unsigned char, 0>))                             \

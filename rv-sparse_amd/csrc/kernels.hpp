@@ -101,9 +101,11 @@ hipError_t launch_analyze(const DevCsr& A, const DevCsr& B, RowStats st,
 hipError_t launch_classify_count(const DevCsr& A, RowStats st, const BinRule& r,
                                  unsigned char* bin_of, int* cnt, hipStream_t s);
 // Fill/numeric phases: classifies by the exact C-row nnz.
+// kept (may be null): rows with kept[i + 1] > kept[i] had their columns kept
+// by the count phase and go to bin 0.
 hipError_t launch_classify_nnz(const DevCsr& A, RowStats st, const int* c_rp,
                                const BinRule& r, unsigned char* bin_of,
-                               hipStream_t s);
+                               hipStream_t s, const long long* kept = nullptr);
 hipError_t launch_partition(int rows, int nbins, PartitionScratch& ps,
                             int* rows_out, hipStream_t s);
 
@@ -117,9 +119,16 @@ hipError_t launch_bin_stats(const int* rows, const int* offsets, int nbins, int 
                             unsigned long long* out, hipStream_t s);
 
 // --- symbolic -----------------------------------------------------------
+// keep (may be null): LDS bitmap rows also write their sorted columns to
+// keep + koff[row] (koff: exclusive scan of min(Op_i, span_i) over them).
 hipError_t launch_count_bin(const BinSpec& b, const int* rows, int nrows,
                             const DevCsr& A, const DevCsr& B, RowStats st,
-                            int* cnt, const GlobalSlots& gs, hipStream_t s);
+                            int* cnt, const GlobalSlots& gs, hipStream_t s,
+                            int* keep = nullptr, const long long* koff = nullptr);
+// Copies the kept columns of the listed rows to C.
+hipError_t launch_copy_kept(const int* rows, int nrows, const int* keep,
+                            const long long* koff, const int* c_rp, int* c_ci,
+                            hipStream_t s);
 hipError_t launch_scan(const int* cnt, int rows, int* c_rp, long long* total,
                        long long* tile_sums, hipStream_t s);
 size_t scan_tile_count(int rows);
@@ -147,9 +156,11 @@ struct MapView {
 };
 // poff = exclusive scan over rows of (bin_of[i] in bin_mask ? Op_i : 0);
 // *total = poff[rows]. `weights` holds rows int64 values of scratch.
+// span_bound: weights min(Op_i, span_i) instead of Op_i.
 hipError_t launch_map_offsets(int rows, const unsigned char* bin_of, unsigned bin_mask,
                               RowStats st, long long* weights, long long* poff,
-                              long long* tile_sums, long long* total, hipStream_t s);
+                              long long* tile_sums, long long* total, hipStream_t s,
+                              bool span_bound = false);
 hipError_t launch_build_map(int width, const int* rows, int nrows, const DevCsr& A,
                             const DevCsr& B, const int* c_rp, const int* c_ci,
                             const long long* poff, void* map, hipStream_t s);
