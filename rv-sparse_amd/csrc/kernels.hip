@@ -1499,64 +1499,89 @@ __global__ __launch_bounds__(kBlock) void k_numeric_rowblock(
   }
 }
 
-// Long rows: one workgroup per (row, tile of T consecutive output
-// positions). The tile's columns and accumulators live in LDS. The row's
+// First index in [lo, hi] of the ascending b_ci[lo, hi) whose value is >= key,
+// searched forward from lo: doubling steps, then halving (cheap when the
+// answer is near lo).
+__device__ __forceinline__ int gm_gallop(const int* __restrict__ b_ci, int lo, int hi,
+                                         int key) {
+  if (lo >= hi || b_ci[lo] >= key) return lo;
+  int bound = 1;
+  while (lo + bound < hi && b_ci[lo + bound] < key) bound <<= 1;
+  return gm_lower_bound(b_ci, lo + (bound >> 1) + 1, min(lo + bound, hi), key);
+}
+
+// Long rows: one workgroup per (row, run of consecutive tiles of T output
+// positions). A tile's columns and accumulators live in LDS. The row's
 // products are streamed in reference order (A entries ascending, then B
 // positions ascending) in groups of one product per lane: every lane finds
 // its product's A entry, gathers the B element and looks its column up in
 // the tile, all lanes in parallel. Only the accumulate is ordered: products
 // of a group that land on the same position are applied in product order,
-// the lowest lane first (an LDS atomicMin picks it per round), so every
-// C(i,j) sees exactly the reference's sequence of separately rounded adds.
-// Tiles of one row share nothing but the A row and B rows they read; a row
-// spanning several tiles first narrows long B rows to the tile's columns.
+// the lowest lane first (an LDS atomicMin on the position's tag picks it per
+// round), so every C(i,j) sees exactly the reference's sequence of separately
+// rounded adds. A row spanning several tiles narrows every B row to each
+// tile's columns; along a workgroup's run of tiles the narrowed range of a B
+// row starts where the previous tile's ended (a per-lane cursor, for A rows
+// of up to kCurChunks * kBlock entries) and its end is found by galloping.
+struct TiledSlot {
+  int tag;    // lowest pending product index of this round (INT_MAX: none)
+  float val;  // accumulator
+};
+
 struct TiledLds {
-  int* cols;
-  float* vals;
-  int* tag;
-  int* pre;    // kBlock + 1 exclusive prefix of the chunk's product counts
-  int* bstart; // kBlock B-row starts
-  float* aval; // kBlock A values
+  TiledSlot* slot;  // T
+  int* cols;        // T
+  int* pre;         // kBlock + 1 exclusive prefix of the chunk's product counts
+  int* bstart;      // kBlock B-row starts
+  float* aval;      // kBlock A values
 };
 
 __device__ __forceinline__ TiledLds tiled_lds(int* lds, int T) {
   TiledLds L;
-  L.cols = lds;
-  L.vals = reinterpret_cast<float*>(lds + T);
-  L.tag = lds + 2 * T;
+  L.slot = reinterpret_cast<TiledSlot*>(lds);
+  L.cols = lds + 2 * T;
   L.pre = lds + 3 * T;
   L.bstart = L.pre + kBlock + 1;
   L.aval = reinterpret_cast<float*>(L.bstart + kBlock);
   return L;
 }
 
+constexpr int kCurChunks = 4;
+
 __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
     const int* __restrict__ rows, int nrows, int T, DevCsr A, DevCsr B,
     const int* __restrict__ c_rp, const int* __restrict__ c_ci,
     float* __restrict__ c_val) {
-  extern __shared__ int lds[];
+  extern __shared__ __attribute__((aligned(16))) int tiled_lds_raw[];
   __shared__ int scan_tmp[kBlock / kWave];
-  const TiledLds L = tiled_lds(lds, T);
+  const TiledLds L = tiled_lds(tiled_lds_raw, T);
   const int tid = threadIdx.x;
   for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
     const int row = rows[r];
     const int c_s = c_rp[row];
     const int n = c_rp[row + 1] - c_s;
     const int ntiles = (n + T - 1) / T;
-    for (int t = blockIdx.y; t < ntiles; t += gridDim.y) {
+    // This workgroup's run of tiles.
+    const int per = (ntiles + gridDim.y - 1) / gridDim.y;
+    const int t_beg = blockIdx.y * per;
+    const int t_end = min(ntiles, t_beg + per);
+    const bool narrow = ntiles > 1;
+    const int a_s = A.rp[row], a_e = A.rp[row + 1];
+    const bool cursors = a_e - a_s <= kCurChunks * kBlock;
+    int cur[kCurChunks];  // per chunk: where this lane's B row continues
+    for (int t = t_beg; t < t_end; ++t) {
       const int t0 = t * T;
       const int tn = min(T, n - t0);
       for (int j = tid; j < tn; j += kBlock) {
         L.cols[j] = c_ci[c_s + t0 + j];
-        L.vals[j] = 0.0f;
-        L.tag[j] = INT_MAX;
+        L.slot[j].tag = INT_MAX;
+        L.slot[j].val = 0.0f;
       }
       __syncthreads();
       const int clo = L.cols[0];
       const int chi = L.cols[tn - 1];
-      const bool narrow = ntiles > 1;
-      const int a_e = A.rp[row + 1];
-      for (int cb = A.rp[row]; cb < a_e; cb += kBlock) {
+      int chunk = 0;
+      for (int cb = a_s; cb < a_e; cb += kBlock, ++chunk) {
         // One A entry per lane: its scale and the part of its B row that
         // can land in this tile.
         const int ap = cb + tid;
@@ -1565,11 +1590,25 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
         if (ap < a_e) {
           const int k = A.ci[ap];
           a = A.val[ap];
-          bs = B.rp[k];
           int be = B.rp[k + 1];
           if (narrow) {
-            bs = gm_lower_bound(B.ci, bs, be, clo);
-            be = gm_lower_bound(B.ci, bs, be, chi + 1);
+            int lo = 0;
+            if (cursors && t > t_beg) {
+#pragma unroll
+              for (int c = 0; c < kCurChunks; ++c)
+                if (c == chunk) lo = cur[c];
+            } else {
+              lo = gm_lower_bound(B.ci, B.rp[k], be, clo);
+            }
+            bs = lo;
+            be = gm_gallop(B.ci, lo, be, chi + 1);
+            if (cursors) {
+#pragma unroll
+              for (int c = 0; c < kCurChunks; ++c)
+                if (c == chunk) cur[c] = be;
+            }
+          } else {
+            bs = B.rp[k];
           }
           len = be - bs;
         }
@@ -1640,20 +1679,25 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
           while (__syncthreads_or(act[0] || act[1])) {
 #pragma unroll
             for (int i = 0; i < kItems; ++i)
-              if (act[i]) atomicMin(&L.tag[pos[i]], i * kBlock + tid);
+              if (act[i]) atomicMin(&L.slot[pos[i]].tag, i * kBlock + tid);
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < kItems; ++i)
-              if (act[i] && L.tag[pos[i]] == i * kBlock + tid) {
-                L.vals[pos[i]] = L.vals[pos[i]] + prod[i];
-                L.tag[pos[i]] = INT_MAX;
-                act[i] = false;
+              if (act[i]) {
+                const TiledSlot v = L.slot[pos[i]];
+                if (v.tag == i * kBlock + tid) {
+                  TiledSlot w;
+                  w.tag = INT_MAX;
+                  w.val = v.val + prod[i];
+                  L.slot[pos[i]] = w;
+                  act[i] = false;
+                }
               }
           }
         }
         __syncthreads();  // the chunk's metadata is replaced next
       }
-      for (int j = tid; j < tn; j += kBlock) c_val[c_s + t0 + j] = L.vals[j];
+      for (int j = tid; j < tn; j += kBlock) c_val[c_s + t0 + j] = L.slot[j].val;
       __syncthreads();  // the tile's LDS is reused by the next tile
     }
   }

@@ -115,6 +115,22 @@ int fixed_class(int phase, int b) {
   }
 }
 
+uint64_t candidate_classes(int phase, int b, bool map_ok);
+
+// The classes the network may pick for bin b (a subset of the tuner's).
+uint64_t selectable_classes(int phase, int b) {
+  // The position map pays off only over repeated numeric-only computes,
+  // which the features cannot see, so its classes are left to the tuner;
+  // the global dense accumulator stays the veto's fallback for the short-row
+  // bins (its cost there grows with B's column count, which the features do
+  // not carry) and a choice for the long-row ones.
+  uint64_t m = candidate_classes(phase, b, false);
+  const int kind = fixed_table(phase).spec[b].kind;
+  if (kind == KIND_HASH || kind == KIND_SEARCH || kind == KIND_DIRECT)
+    m &= ~(1ull << kClsGlobal);
+  return m;
+}
+
 uint64_t candidate_classes(int phase, int b, bool map_ok) {
   const PhaseTable& t = fixed_table(phase);
   if (b < 1 || b >= t.rule.nbins) return 0;
@@ -371,8 +387,9 @@ constexpr double kTau = 0.1;
 void sample_targets(const rvsp_learned_sample_t& smp, float* q, unsigned long long* mask) {
   double best = 0;
   *mask = 0;
+  const uint64_t sel = selectable_classes(smp.phase, smp.bin);
   for (int c = 0; c < kMlpK; ++c)
-    if (smp.us[c] >= 0.f && class_entry(c).kind != KIND_MAP) {
+    if (smp.us[c] >= 0.f && ((sel >> c) & 1ull)) {
       *mask |= 1ull << c;
       if (best == 0 || smp.us[c] < best) best = smp.us[c];
     }
@@ -406,9 +423,7 @@ rvsp_status_t select_learned(PhaseTable& t, int phase, const std::vector<BinFeat
   std::vector<unsigned long long> mask;
   for (int b = 1; b < t.rule.nbins && b < (int)f.size(); ++b) {
     if (f[b].rows <= 0) continue;
-    // The position map pays off only over repeated numeric-only computes,
-    // which the features cannot see; its classes are left to the tuner.
-    uint64_t m = candidate_classes(phase, b, false);
+    uint64_t m = selectable_classes(phase, b);
     for (int c = 0; c < kMlpK; ++c) {
       if (!((m >> c) & 1ull)) continue;
       PhaseTable trial = t;

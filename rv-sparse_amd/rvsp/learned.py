@@ -101,10 +101,17 @@ def load_samples(path):
 
 
 def candidate_masks(sm, include_map=False):
-    """Per-sample allowed-class bit sets from the measured classes."""
+    """Per-sample bit sets of the measured classes the network may pick (as
+    the library's selection: no position-map classes, and no global dense
+    accumulator for bins whose fixed kind is a short-row kernel)."""
+    tab = class_table()
     ok = sm["us"] >= 0
     if not include_map:
         ok = ok & ~np.array([(map_class_mask() >> c) & 1 for c in range(NK)], bool)[None, :]
+    short = np.array([tab[c]["kind"] in (F.KIND_HASH, F.KIND_SEARCH, F.KIND_DIRECT)
+                      for c in sm["fixed_class"]], bool)
+    glob = np.array([t["kind"] == F.KIND_GLOBAL for t in tab], bool)
+    ok = ok & ~(short[:, None] & glob[None, :])
     return (ok.astype(np.uint64) << np.arange(NK, dtype=np.uint64)[None, :]).sum(
         axis=1, dtype=np.uint64)
 
