@@ -91,10 +91,10 @@ __device__ __forceinline__ int group_min(int v) {
   return v;
 }
 
-// Block-wide exclusive scan for blockDim.x == kBlock: wave scans plus an LDS
-// carry. `tmp` holds kBlock / kWave values. Returns the exclusive prefix and
+// Block-wide exclusive scan for blockDim.x == NT: wave scans plus an LDS
+// carry. `tmp` holds NT / kWave values. Returns the exclusive prefix and
 // the block total in *total. Must be reached by every thread of the block.
-template <typename T>
+template <typename T, int NT = kBlock>
 __device__ __forceinline__ T block_excl_scan(T v, T* tmp, T* total) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x >> 6;
@@ -103,7 +103,7 @@ __device__ __forceinline__ T block_excl_scan(T v, T* tmp, T* total) {
   __syncthreads();
   T carry = 0, tot = 0;
 #pragma unroll
-  for (int w = 0; w < kBlock / kWave; ++w) {
+  for (int w = 0; w < NT / kWave; ++w) {
     T s = tmp[w];
     carry += (w < wid) ? s : T(0);
     tot += s;

@@ -63,10 +63,10 @@ rvsp_status_t device_ready() {
     // Dynamic LDS above the default needs an opt-in per kernel; the limit
     // the plans may use is what the device grants a workgroup, less the
     // kernels' static LDS.
-    const int limit = (int)prop.sharedMemPerBlock - 8 * 1024;
+    const int limit = (int)prop.sharedMemPerBlock - kStaticLdsReserve;
     if (init_kernel_attributes(limit) != hipSuccess) {
       (void)hipGetLastError();
-      set_lds_limit(64 * 1024 - 8 * 1024);
+      set_lds_limit(64 * 1024 - kStaticLdsReserve);
     } else {
       set_lds_limit(limit);
     }

@@ -66,23 +66,24 @@ __device__ __forceinline__ void group_for_each_col(int a_s, int a_e,
   }
 }
 
-// The last entry e >= j of a kBlock-long nondecreasing prefix array with
+// The last entry e >= j of an N-long nondecreasing prefix array with
 // pre[e] <= p, given pre[j] <= p: gallop forward, then halve.
+template <int N = kBlock>
 __device__ __forceinline__ int advance_entry(const int* pre, int j, int p) {
   int step = 1;
-  while (j + step < kBlock && pre[j + step] <= p) {
+  while (j + step < N && pre[j + step] <= p) {
     j += step;
     step <<= 1;
   }
   for (step >>= 1; step > 0; step >>= 1)
-    if (j + step < kBlock && pre[j + step] <= p) j += step;
+    if (j + step < N && pre[j + step] <= p) j += step;
   return j;
 }
 
-// Whole-workgroup version of group_for_each_col (blockDim.x == kBlock).
-// `sh_excl` and `sh_bs` hold kBlock ints each, `tmp` kBlock / kWave ints.
+// Whole-workgroup version of group_for_each_col (blockDim.x == NT).
+// `sh_excl` and `sh_bs` hold NT ints each, `tmp` NT / kWave ints.
 // Must be reached by every thread of the block.
-template <class F>
+template <int NT, class F>
 __device__ __forceinline__ void block_for_each_col(int a_s, int a_e,
                                                    const int* __restrict__ a_ci,
                                                    const int* __restrict__ b_rp,
@@ -90,7 +91,7 @@ __device__ __forceinline__ void block_for_each_col(int a_s, int a_e,
                                                    int* sh_excl, int* sh_bs,
                                                    int* tmp, F&& f) {
   const int t = threadIdx.x;
-  for (int base = a_s; base < a_e; base += kBlock) {
+  for (int base = a_s; base < a_e; base += NT) {
     const int ap = base + t;
     int bs = 0, len = 0;
     if (ap < a_e) {
@@ -99,26 +100,26 @@ __device__ __forceinline__ void block_for_each_col(int a_s, int a_e,
       len = b_rp[k + 1] - bs;
     }
     int total;
-    const int excl = block_excl_scan(len, tmp, &total);
+    const int excl = block_excl_scan<int, NT>(len, tmp, &total);
     sh_excl[t] = excl;
     sh_bs[t] = bs;
     __syncthreads();
-    // Two interleaved streams per lane (products t + 2k kBlock and
-    // t + (2k+1) kBlock), so two gathers are in flight; each stream keeps the
+    // Two interleaved streams per lane (products t + 2k NT and
+    // t + (2k+1) NT), so two gathers are in flight; each stream keeps the
     // entry of its last product and gallops forward from it.
     // The loop is uniform over the block (f sees every lane, with a flag for
     // lanes past the end), so f may use cross-lane operations.
     int ja = 0, jb = 0;
-    for (int p0 = 0; p0 < total; p0 += 2 * kBlock) {
+    for (int p0 = 0; p0 < total; p0 += 2 * NT) {
       const int p = p0 + t;
-      const int pb = p + kBlock;
+      const int pb = p + NT;
       int ca = -1, cb = -1;
       if (p < total) {
-        ja = advance_entry(sh_excl, ja, p);
+        ja = advance_entry<NT>(sh_excl, ja, p);
         ca = b_ci[sh_bs[ja] + (p - sh_excl[ja])];
       }
       if (pb < total) {
-        jb = advance_entry(sh_excl, max(ja, jb), pb);
+        jb = advance_entry<NT>(sh_excl, max(ja, jb), pb);
         cb = b_ci[sh_bs[jb] + (pb - sh_excl[jb])];
       }
       f(ca, p < total);
@@ -480,21 +481,24 @@ __global__ __launch_bounds__(kBlock) void k_count_hash(
 // also keep the sorted columns at koff[row] of a side buffer, so the fill
 // phase copies them instead of building the bitmap again.
 constexpr int kBitmapCount = 0, kBitmapFill = 1, kBitmapCountKeep = 2;
+constexpr int kWideBlock = 1024;              // threads of a workgroup holding a large bitmap
+constexpr int kWideBitmapBytes = 80 * 1024;   // bitmaps above this fit once per CU
 
 // One workgroup per row. The row's columns lie in [lo, hi]; a bitmap over that
 // span lives in LDS (GLOBAL_BM == false, span <= 32 * words) or in this
 // workgroup's slot of global scratch (GLOBAL_BM == true, absolute column
 // bits, words cleared again after use). FILL == false counts set bits;
 // FILL == true writes the set columns in ascending order to C.
-template <bool GLOBAL_BM, int MODE>
-__global__ __launch_bounds__(kBlock) void k_bitmap_row(
+template <bool GLOBAL_BM, int MODE, int NT = kBlock>
+__global__ __launch_bounds__(NT) void k_bitmap_row(
     const int* __restrict__ rows, int nrows, DevCsr A, DevCsr B, RowStats st,
     int* __restrict__ cnt, const int* __restrict__ c_rp, int* __restrict__ c_ci,
     unsigned* __restrict__ gbits, size_t slot_words, int* __restrict__ keep,
     const long long* __restrict__ koff) {
   constexpr bool FILL = MODE == kBitmapFill;
+  constexpr int kWaves = NT / kWave;
   extern __shared__ unsigned lbits[];
-  __shared__ int sh_excl[kBlock], sh_bs[kBlock], tmp[kBlock / kWave];
+  __shared__ int sh_excl[NT], sh_bs[NT], tmp[kWaves];
   unsigned* bits = GLOBAL_BM ? gbits + (size_t)blockIdx.x * slot_words : lbits;
   for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
     const int row = rows[r];
@@ -504,12 +508,12 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_row(
     const int w1 = GLOBAL_BM ? (hi >> 5) + 1 : ((hi - lo) >> 5) + 1;
     const int base_col = GLOBAL_BM ? 0 : lo;
     if (!GLOBAL_BM) {
-      for (int w = threadIdx.x; w < w1; w += kBlock) bits[w] = 0u;
+      for (int w = threadIdx.x; w < w1; w += NT) bits[w] = 0u;
       __syncthreads();
     }
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x >> 6;
-    block_for_each_col(A.rp[row], A.rp[row + 1], A.ci, B.rp, B.ci, sh_excl,
+    block_for_each_col<NT>(A.rp[row], A.rp[row + 1], A.ci, B.rp, B.ci, sh_excl,
                        sh_bs, tmp, [&](int col, bool valid) {
                          if (valid) {
                            const int d = col - base_col;
@@ -520,11 +524,11 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_row(
                        });
     // block_for_each_col ends with a barrier, which also orders the global
     // atomics above before the reads below (workgroup scope).
-    // The span's words are split into one quarter per wave; a wave reads its
-    // quarter 64 consecutive words at a time (no bank conflicts).
+    // The span's words are split into one part per wave; a wave reads its
+    // part 64 consecutive words at a time (no bank conflicts).
     const int nw = w1 - w0;
-    const int q0 = w0 + (int)((long long)nw * wid / (kBlock / kWave));
-    const int q1 = w0 + (int)((long long)nw * (wid + 1) / (kBlock / kWave));
+    const int q0 = w0 + (int)((long long)nw * wid / kWaves);
+    const int q1 = w0 + (int)((long long)nw * (wid + 1) / kWaves);
     int c = 0;
     for (int w = q0 + lane; w < q1; w += kWave) {
       c += __popc(bits[w]);
@@ -535,7 +539,7 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_row(
     __syncthreads();
     int total = 0, base = 0;
 #pragma unroll
-    for (int v = 0; v < kBlock / kWave; ++v) {
+    for (int v = 0; v < kWaves; ++v) {
       total += tmp[v];
       base += v < wid ? tmp[v] : 0;
     }
@@ -596,7 +600,19 @@ hipError_t launch_count_bin(const BinSpec& b, const int* rows, int nrows,
         default: return hipErrorInvalidValue;
       }
     case KIND_BITMAP:
-      if (keep) {
+      // A bitmap that leaves room for one workgroup per CU gets 16 waves.
+      if (b.lds_bytes > kWideBitmapBytes) {
+        if (keep)
+          hipLaunchKernelGGL((k_bitmap_row<false, kBitmapCountKeep, kWideBlock>),
+                             dim3(row_grid(nrows)), dim3(kWideBlock), b.lds_bytes, s, rows,
+                             nrows, A, B, st, cnt, nullptr, nullptr, nullptr, (size_t)0, keep,
+                             koff);
+        else
+          hipLaunchKernelGGL((k_bitmap_row<false, kBitmapCount, kWideBlock>),
+                             dim3(row_grid(nrows)), dim3(kWideBlock), b.lds_bytes, s, rows,
+                             nrows, A, B, st, cnt, nullptr, nullptr, nullptr, (size_t)0,
+                             nullptr, nullptr);
+      } else if (keep) {
         hipLaunchKernelGGL((k_bitmap_row<false, kBitmapCountKeep>), dim3(row_grid(nrows)),
                            dim3(kBlock), b.lds_bytes, s, rows, nrows, A, B, st, cnt,
                            nullptr, nullptr, nullptr, (size_t)0, keep, koff);
@@ -801,9 +817,14 @@ hipError_t launch_fill_bin(const BinSpec& b, const int* rows, int nrows,
         default: return hipErrorInvalidValue;
       }
     case KIND_BITMAP:
-      hipLaunchKernelGGL((k_bitmap_row<false, kBitmapFill>), dim3(row_grid(nrows)),
-                         dim3(kBlock), b.lds_bytes, s, rows, nrows, A, B, st,
-                         nullptr, c_rp, c_ci, nullptr, (size_t)0, nullptr, nullptr);
+      if (b.lds_bytes > kWideBitmapBytes)
+        hipLaunchKernelGGL((k_bitmap_row<false, kBitmapFill, kWideBlock>), dim3(row_grid(nrows)),
+                           dim3(kWideBlock), b.lds_bytes, s, rows, nrows, A, B, st,
+                           nullptr, c_rp, c_ci, nullptr, (size_t)0, nullptr, nullptr);
+      else
+        hipLaunchKernelGGL((k_bitmap_row<false, kBitmapFill>), dim3(row_grid(nrows)),
+                           dim3(kBlock), b.lds_bytes, s, rows, nrows, A, B, st,
+                           nullptr, c_rp, c_ci, nullptr, (size_t)0, nullptr, nullptr);
       return hipGetLastError();
     case KIND_GLOBAL:
       if (!gs.base || gs.nslots <= 0) return hipErrorInvalidValue;
@@ -1513,40 +1534,78 @@ __device__ __forceinline__ int gm_gallop(const int* __restrict__ b_ci, int lo, i
 // Long rows: one workgroup per (row, run of consecutive tiles of T output
 // positions). A tile's columns and accumulators live in LDS. The row's
 // products are streamed in reference order (A entries ascending, then B
-// positions ascending) in groups of one product per lane: every lane finds
-// its product's A entry, gathers the B element and looks its column up in
-// the tile, all lanes in parallel. Only the accumulate is ordered: products
-// of a group that land on the same position are applied in product order,
-// the lowest lane first (an LDS atomicMin on the position's tag picks it per
-// round), so every C(i,j) sees exactly the reference's sequence of separately
+// positions ascending) in groups of kTiledItems products per lane: every lane
+// finds its products' A entries, gathers the B elements and looks their
+// columns up in the tile, all lanes in parallel. Only the accumulate is
+// ordered. Each wave owns every fourth 32-position stripe of the tile; a
+// group's products are routed to the owner's list in product order (a stable
+// split through LDS), and each wave applies its list 64 products at a time:
+// products of one batch that land on the same position are applied lowest
+// lane first (an LDS atomicMin on the position's tag picks it per round; one
+// wave's LDS operations execute in program order, so no barrier is needed).
+// Every C(i,j) thus sees exactly the reference's sequence of separately
 // rounded adds. A row spanning several tiles narrows every B row to each
 // tile's columns; along a workgroup's run of tiles the narrowed range of a B
 // row starts where the previous tile's ended (a per-lane cursor, for A rows
 // of up to kCurChunks * kBlock entries) and its end is found by galloping.
 struct TiledSlot {
-  int tag;    // lowest pending product index of this round (INT_MAX: none)
+  int tag;    // lane of the lowest pending product of this round (INT_MAX: none)
   float val;  // accumulator
 };
 
+constexpr int kTiledItems = 2;             // products per lane per group
+constexpr int kTiledGroup = kTiledItems * kBlock;
+constexpr int kTiledWaves = kBlock / kWave;
+
+constexpr int kEntrySlots = 1024;  // entry table of a chunk (see k_numeric_tiled)
+constexpr int kPrePad = 8;         // entries probed past a table hit
+
 struct TiledLds {
+  int* cnt;         // [item][wave][owner] products of a group
   TiledSlot* slot;  // T
-  int* cols;        // T
-  int* pre;         // kBlock + 1 exclusive prefix of the chunk's product counts
+  int2* stage;      // kTiledGroup (position, product bits), grouped by owner wave
+  int* cols;        // T: the tile's columns, or its bitmap index (uint2 per 32 columns)
+  int* etab;        // kEntrySlots: entry of product m << shift
+  int* pre;         // kBlock + kPrePad exclusive prefix of the chunk's product counts
   int* bstart;      // kBlock B-row starts
   float* aval;      // kBlock A values
 };
 
 __device__ __forceinline__ TiledLds tiled_lds(int* lds, int T) {
   TiledLds L;
-  L.slot = reinterpret_cast<TiledSlot*>(lds);
-  L.cols = lds + 2 * T;
-  L.pre = lds + 3 * T;
-  L.bstart = L.pre + kBlock + 1;
+  L.cnt = lds;  // first: 16-byte aligned for int4 reads; slot and stage stay 8-byte aligned for any T
+  L.slot = reinterpret_cast<TiledSlot*>(lds + kTiledItems * kTiledWaves * kTiledWaves);
+  L.stage = reinterpret_cast<int2*>(L.slot + T);
+  L.cols = reinterpret_cast<int*>(L.stage + kTiledGroup);
+  L.etab = L.cols + T;
+  L.pre = L.etab + kEntrySlots;
+  L.bstart = L.pre + kBlock + kPrePad;
   L.aval = reinterpret_cast<float*>(L.bstart + kBlock);
   return L;
 }
 
 constexpr int kCurChunks = 4;
+
+// The A entry of product q of a chunk (the last e with pre[e] <= q): the
+// entry table gives the entry of the slot's first product, the kPrePad next
+// prefixes are compared at once, and galloping finishes the rare longer step.
+__device__ __forceinline__ int tiled_entry(const TiledLds& L, int shift, int q) {
+  int e = L.etab[q >> shift];
+  int c = 0;
+#pragma unroll
+  for (int j = 1; j <= kPrePad; ++j) c += L.pre[e + j] <= q ? 1 : 0;
+  e += c;
+  if (c == kPrePad) e = advance_entry(L.pre, e, q);
+  return e;
+}
+
+// Position of column col (clo <= col <= chi, present) in the tile, from the
+// bitmap index: per 32 columns the set bits and the positions before them.
+__device__ __forceinline__ int tiled_index_pos(const uint2* idx, int clo, int col) {
+  const int d = col - clo;
+  const uint2 v = idx[d >> 5];
+  return (int)v.y + __popc(v.x & ((1u << (d & 31)) - 1u));
+}
 
 __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
     const int* __restrict__ rows, int nrows, int T, DevCsr A, DevCsr B,
@@ -1556,6 +1615,10 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
   __shared__ int scan_tmp[kBlock / kWave];
   const TiledLds L = tiled_lds(tiled_lds_raw, T);
   const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wid = tid >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  if (tid < kPrePad) L.pre[kBlock + tid] = INT_MAX;  // probes past the chunk end
   for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
     const int row = rows[r];
     const int c_s = c_rp[row];
@@ -1572,14 +1635,42 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
     for (int t = t_beg; t < t_end; ++t) {
       const int t0 = t * T;
       const int tn = min(T, n - t0);
+      const int clo = c_ci[c_s + t0];
+      const int chi = c_ci[c_s + t0 + tn - 1];
+      // Position lookup: a bitmap index over the tile's column span when it
+      // fits the column area (two words per 32 columns), else binary search.
+      const int iw = ((chi - clo) >> 5) + 1;
+      const bool use_idx = iw <= T / 2;
+      uint2* idx = reinterpret_cast<uint2*>(L.cols);
+      if (use_idx) {
+        for (int w = tid; w < iw; w += kBlock) idx[w] = make_uint2(0u, 0u);
+      }
       for (int j = tid; j < tn; j += kBlock) {
-        L.cols[j] = c_ci[c_s + t0 + j];
         L.slot[j].tag = INT_MAX;
         L.slot[j].val = 0.0f;
       }
       __syncthreads();
-      const int clo = L.cols[0];
-      const int chi = L.cols[tn - 1];
+      if (use_idx) {
+        for (int j = tid; j < tn; j += kBlock) {
+          const int d = c_ci[c_s + t0 + j] - clo;
+          atomicOr(&idx[d >> 5].x, 1u << (d & 31));
+        }
+        __syncthreads();
+        // Positions before each word: per thread a run of consecutive words.
+        const int per_t = (iw + kBlock - 1) / kBlock;
+        const int w_beg = min(iw, tid * per_t), w_end = min(iw, w_beg + per_t);
+        int cnt_t = 0;
+        for (int w = w_beg; w < w_end; ++w) cnt_t += __popc(idx[w].x);
+        int tot_w;
+        int run = block_excl_scan(cnt_t, scan_tmp, &tot_w);
+        for (int w = w_beg; w < w_end; ++w) {
+          idx[w].y = (unsigned)run;
+          run += __popc(idx[w].x);
+        }
+      } else {
+        for (int j = tid; j < tn; j += kBlock) L.cols[j] = c_ci[c_s + t0 + j];
+      }
+      __syncthreads();
       int chunk = 0;
       for (int cb = a_s; cb < a_e; cb += kBlock, ++chunk) {
         // One A entry per lane: its scale and the part of its B row that
@@ -1617,82 +1708,136 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
         L.pre[tid] = excl;
         L.bstart[tid] = bs;
         L.aval[tid] = a;
+        // Entry table: slot m holds the entry of product m << shift.
+        int shift = 5;
+        while ((total >> shift) >= kEntrySlots) ++shift;
+        {
+          const int step = 1 << shift;
+          for (int m = (excl + step - 1) >> shift; (m << shift) < excl + len; ++m)
+            L.etab[m] = tid;
+        }
         __syncthreads();
-        // Groups of kItems * kBlock products, kItems per lane (products
+        // Groups of kTiledGroup products, kTiledItems per lane (products
         // q0 + i kBlock + tid), software-pipelined: the next group's gathers
         // are issued before this group's lookups and adds.
-        constexpr int kItems = 2;
-        int e = 0;
-        int n_col[kItems];
-        float n_b[kItems], n_a[kItems];
+        int n_col[kTiledItems];
+        float n_b[kTiledItems], n_a[kTiledItems];
 #pragma unroll
-        for (int i = 0; i < kItems; ++i) {
+        for (int i = 0; i < kTiledItems; ++i) {
           const int q = i * kBlock + tid;
           n_col[i] = -1;
           n_b[i] = n_a[i] = 0.0f;
           if (q < total) {
-            e = advance_entry(L.pre, e, q);
+            const int e = tiled_entry(L, shift, q);
             const int bi = L.bstart[e] + (q - L.pre[e]);
             n_col[i] = B.ci[bi];
             n_b[i] = B.val[bi];
             n_a[i] = L.aval[e];
           }
         }
-        for (int q0 = 0; q0 < total; q0 += kItems * kBlock) {
-          int col[kItems];
-          float bv[kItems], av[kItems];
+        for (int q0 = 0; q0 < total; q0 += kTiledGroup) {
+          int col[kTiledItems];
+          float bv[kTiledItems], av[kTiledItems];
 #pragma unroll
-          for (int i = 0; i < kItems; ++i) {
+          for (int i = 0; i < kTiledItems; ++i) {
             col[i] = n_col[i];
             bv[i] = n_b[i];
             av[i] = n_a[i];
           }
 #pragma unroll
-          for (int i = 0; i < kItems; ++i) {
-            // The A entry of product qn: the last e with pre[e] <= qn
-            // (entries without products share their successor's prefix).
-            const int qn = q0 + (kItems + i) * kBlock + tid;
+          for (int i = 0; i < kTiledItems; ++i) {
+            // The A entry of product qn (entries without products share
+            // their successor's prefix).
+            const int qn = q0 + (kTiledItems + i) * kBlock + tid;
             n_col[i] = -1;
             if (qn < total) {
-              e = advance_entry(L.pre, e, qn);
+              const int e = tiled_entry(L, shift, qn);
               const int bi = L.bstart[e] + (qn - L.pre[e]);
               n_col[i] = B.ci[bi];
               n_b[i] = B.val[bi];
               n_a[i] = L.aval[e];
             }
           }
-          int pos[kItems];
-          float prod[kItems];
-          bool act[kItems];
+          int pos[kTiledItems], own[kTiledItems], rank[kTiledItems];
+          float prod[kTiledItems];
 #pragma unroll
-          for (int i = 0; i < kItems; ++i) {
+          for (int i = 0; i < kTiledItems; ++i) {
             pos[i] = -1;
             prod[i] = 0.0f;
+            own[i] = kTiledWaves;  // no product
             if (col[i] >= clo && col[i] <= chi) {  // col -1: no product
-              pos[i] = lds_find(L.cols, tn, col[i]);
+              pos[i] = use_idx ? tiled_index_pos(idx, clo, col[i]) : lds_find(L.cols, tn, col[i]);
               prod[i] = av[i] * bv[i];  // rounded here, added below
+              own[i] = (pos[i] >> 5) & (kTiledWaves - 1);
             }
-            act[i] = pos[i] >= 0;
           }
-          // Ordered adds: per round, the lowest product index among those
-          // still pending on a position wins it.
-          while (__syncthreads_or(act[0] || act[1])) {
+          // Stable split by owner: per (item, wave) the count for every
+          // owner, and each product's rank among its wave's products with
+          // the same owner. Counts are stored owner-major, (item, wave) minor,
+          // so their exclusive scan is every (owner, item, wave) run's offset
+          // in the staging area: owner 0's list first, each list in (item,
+          // wave, lane) order, which is product order.
+          const unsigned long long has0 = __ballot(own[0] < kTiledWaves);
+          const unsigned long long lo0 = __ballot(own[0] & 1), hi0 = __ballot(own[0] & 2);
+          const unsigned long long has1 = __ballot(own[1] < kTiledWaves);
+          const unsigned long long lo1 = __ballot(own[1] & 1), hi1 = __ballot(own[1] & 2);
 #pragma unroll
-            for (int i = 0; i < kItems; ++i)
-              if (act[i]) atomicMin(&L.slot[pos[i]].tag, i * kBlock + tid);
-            __syncthreads();
+          for (int i = 0; i < kTiledItems; ++i) {
+            const unsigned long long has = i ? has1 : has0;
+            const unsigned long long lo = i ? lo1 : lo0, hi = i ? hi1 : hi0;
+            if (lane < kTiledWaves) {
+              const unsigned long long m =
+                  has & ((lane & 1) ? lo : ~lo) & ((lane & 2) ? hi : ~hi);
+              L.cnt[lane * kTiledItems * kTiledWaves + i * kTiledWaves + wid] = __popcll(m);
+            }
+            const int o = own[i] & (kTiledWaves - 1);
+            const unsigned long long mine = has & ((o & 1) ? lo : ~lo) & ((o & 2) ? hi : ~hi);
+            rank[i] = __popcll(mine & below);
+          }
+          static_assert(kTiledItems == 2 && kTiledWaves == 4, "split is written for 2 x 4");
+          __syncthreads();
+          constexpr int kRuns = kTiledWaves * kTiledItems * kTiledWaves;  // 32
+          const int cv = lane < kRuns ? L.cnt[lane] : 0;
+          const int run_incl = group_incl_scan<kWave>(cv);
+          const int run_off = run_incl - cv;
 #pragma unroll
-            for (int i = 0; i < kItems; ++i)
-              if (act[i]) {
-                const TiledSlot v = L.slot[pos[i]];
-                if (v.tag == i * kBlock + tid) {
+          for (int i = 0; i < kTiledItems; ++i) {
+            const int o = own[i] & (kTiledWaves - 1);
+            const int off = __shfl(run_off, o * kTiledItems * kTiledWaves + i * kTiledWaves + wid);
+            if (own[i] < kTiledWaves)
+              L.stage[off + rank[i]] = make_int2(pos[i], __float_as_int(prod[i]));
+          }
+          // This wave's list: positions in its stripes only, so no other
+          // wave touches them; 64 products per batch, in order.
+          const int my_base = __shfl(run_off, wid * kTiledItems * kTiledWaves);
+          const int my_end = wid + 1 < kTiledWaves
+                                 ? __shfl(run_off, (wid + 1) * kTiledItems * kTiledWaves)
+                                 : __shfl(run_incl, kRuns - 1);
+          const int my_n = my_end - my_base;
+          __syncthreads();
+          for (int k0 = 0; k0 < my_n; k0 += kWave) {
+            const int j = k0 + lane;
+            bool act = j < my_n;
+            int p = 0;
+            float v = 0.0f;
+            if (act) {
+              const int2 sv = L.stage[my_base + j];
+              p = sv.x;
+              v = __int_as_float(sv.y);
+            }
+            while (__any(act)) {
+              if (act) atomicMin(&L.slot[p].tag, lane);
+              if (act) {
+                const TiledSlot cur_s = L.slot[p];
+                if (cur_s.tag == lane) {
                   TiledSlot w;
                   w.tag = INT_MAX;
-                  w.val = v.val + prod[i];
-                  L.slot[pos[i]] = w;
-                  act[i] = false;
+                  w.val = cur_s.val + v;
+                  L.slot[p] = w;
+                  act = false;
                 }
               }
+            }
           }
         }
         __syncthreads();  // the chunk's metadata is replaced next
@@ -1703,7 +1848,10 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
   }
 }
 
-int tiled_lds_bytes(int T) { return (3 * T + 3 * kBlock + 1) * 4; }
+int tiled_lds_bytes(int T) {
+  return (3 * T + 2 * kTiledGroup + kTiledItems * kTiledWaves * kTiledWaves +
+          kEntrySlots + 3 * kBlock + kPrePad) * 4;
+}
 
 template <int G>
 static hipError_t launch_numeric_g(const BinSpec& b, const int* rows, int nrows,
@@ -1815,6 +1963,9 @@ hipError_t init_kernel_attributes(int lds_bytes) {
   RVSP_ALLOW((k_bitmap_row<false, kBitmapCount>))
   RVSP_ALLOW((k_bitmap_row<false, kBitmapFill>))
   RVSP_ALLOW((k_bitmap_row<false, kBitmapCountKeep>))
+  RVSP_ALLOW((k_bitmap_row<false, kBitmapCount, kWideBlock>))
+  RVSP_ALLOW((k_bitmap_row<false, kBitmapFill, kWideBlock>))
+  RVSP_ALLOW((k_bitmap_row<false, kBitmapCountKeep, kWideBlock>))
   RVSP_ALLOW(k_numeric_rowblock<false>) RVSP_ALLOW(k_numeric_tiled)
 #define RVSP_ALLOW_MAP(G)                                                     \
   RVSP_ALLOW((k_numeric_map<G, unsigned char, 0>))                             \

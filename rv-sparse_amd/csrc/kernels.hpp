@@ -147,6 +147,8 @@ hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
 
 // Dynamic LDS of the tiled long-row numeric kernel for tiles of T positions.
 int tiled_lds_bytes(int T);
+// Default tile of the long-row numeric kernel: three workgroups fit a CU.
+constexpr int kTiledPositions = 3072;
 
 // --- position map (numeric structure cache) -------------------------------
 struct MapView {
@@ -173,6 +175,9 @@ hipError_t init_kernel_attributes(int lds_bytes);
 
 // Largest dynamic LDS a workgroup may request on gfx950.
 constexpr int kMaxLdsBytes = 160 * 1024;
+// LDS kept back from plans for the kernels' static arrays (the largest is the
+// 1024-thread bitmap kernel's, about 8.3 KB).
+constexpr int kStaticLdsReserve = 12 * 1024;
 
 // A entries per row the SEARCH kernel's segment mode stages in LDS (a
 // workgroup whose rows hold more A entries reads them from global memory).

@@ -175,7 +175,7 @@ BinChoice class_choice(int cls, int phase, int b, int max_metric) {
       ch.cap = fs.kind == KIND_SEARCH ? fs.cap : next_pow2(std::max(16LL, need));
       break;
     case KIND_BLOCK:
-      ch.cap = 4096;  // positions per tile
+      ch.cap = kTiledPositions;  // positions per tile
       break;
     case KIND_BITMAP:
       ch.cap = fs.cap;

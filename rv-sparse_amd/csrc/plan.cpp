@@ -10,9 +10,8 @@ namespace rvsp {
 
 namespace {
 
-constexpr int kTile = 4096;  // positions per tile of the long-row numeric kernel
 
-int g_lds_limit = kMaxLdsBytes - 8 * 1024;  // dynamic part, set per device
+int g_lds_limit = kMaxLdsBytes - kStaticLdsReserve;  // dynamic part, set per device
 
 bool pow2(int x) { return x > 0 && (x & (x - 1)) == 0; }
 
@@ -90,7 +89,7 @@ int lds_bytes_for(int phase, int kind, int group, int cap, int variant) {
     default:
       break;
   }
-  if (bytes > kMaxLdsBytes - 8 * 1024) return -1;
+  if (bytes > kMaxLdsBytes - kStaticLdsReserve) return -1;
   return (int)bytes;
 }
 
@@ -138,10 +137,11 @@ Strategy fixed_strategy() {
     const int G[] = {4, 8, 16, 32, 64, 64, 64, 64};
     for (int i = 0; i < 8; ++i)
       set_bin(t, PHASE_NUMERIC, b++, KIND_SEARCH, G[i], C[i], C[i], INT_MAX);
-    // Longer rows: tiles of 4096 positions, one workgroup per tile.
-    set_bin(t, PHASE_NUMERIC, b++, KIND_BLOCK, kBlock, kTile, 4096, INT_MAX);
-    set_bin(t, PHASE_NUMERIC, b++, KIND_BLOCK, kBlock, kTile, 16384, INT_MAX);
-    set_bin(t, PHASE_NUMERIC, b++, KIND_BLOCK, kBlock, kTile, INT_MAX, INT_MAX);
+    // Longer rows: tiles of kTiledPositions positions, runs of tiles per
+    // workgroup.
+    set_bin(t, PHASE_NUMERIC, b++, KIND_BLOCK, kBlock, kTiledPositions, 4096, INT_MAX);
+    set_bin(t, PHASE_NUMERIC, b++, KIND_BLOCK, kBlock, kTiledPositions, 16384, INT_MAX);
+    set_bin(t, PHASE_NUMERIC, b++, KIND_BLOCK, kBlock, kTiledPositions, INT_MAX, INT_MAX);
     t.rule.nbins = b;
   }
   return s;
