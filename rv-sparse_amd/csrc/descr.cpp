@@ -134,6 +134,7 @@ void drop_structure(rvsp_spgemm_descr* d) {
   d->map_bins = 0;
   d->map_width = 0;
   d->map_promoted = false;
+  d->split_valid = false;
   d->keep_ok = false;
   d->keep_bins = 0;
   d->estimated = 0;
@@ -417,9 +418,49 @@ rvsp_status_t run_fill(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
   return RVSP_SUCCESS;
 }
 
+rvsp_status_t ensure_split(rvsp_spgemm_descr* d, const DevCsr& B, hipStream_t s) {
+  if (d->split_valid) return RVSP_SUCCESS;
+  d->split = SplitTable();
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  RVSP_HIP_TRY(hipStreamIsCapturing(s, &cap));
+  if (cap != hipStreamCaptureStatusNone) return RVSP_SUCCESS;  // searches run on B
+  if (B.rows <= 0 || B.nnz < kSplitMin) {
+    d->split_valid = true;  // no row is long enough
+    return RVSP_SUCCESS;
+  }
+  RVSP_TRY(ensure(d->split_cnt, (size_t)B.rows * sizeof(int)));
+  RVSP_TRY(ensure(d->split_off, ((size_t)B.rows + 1) * sizeof(int)));
+  RVSP_TRY(ensure(d->scan_tiles, scan_tile_count(B.rows) * sizeof(long long)));
+  RVSP_TRY(ensure(d->total, sizeof(long long)));
+  RVSP_HIP_TRY(launch_split_counts(B, (int*)d->split_cnt.p, s));
+  RVSP_HIP_TRY(launch_scan((const int*)d->split_cnt.p, B.rows, (int*)d->split_off.p,
+                           (long long*)d->total.p, (long long*)d->scan_tiles.p, s));
+  long long entries = 0;
+  RVSP_HIP_TRY(hipMemcpyAsync(&entries, d->total.p, sizeof(entries), hipMemcpyDeviceToHost, s));
+  RVSP_HIP_TRY(hipStreamSynchronize(s));
+  // The table is an accelerator: without room for it the searches run on B.
+  if (entries <= 0 || entries > INT32_MAX / 2 ||
+      ensure(d->split_sp, (size_t)entries * sizeof(int)) != RVSP_SUCCESS) {
+    d->split_valid = true;
+    return RVSP_SUCCESS;
+  }
+  SplitTable t;
+  t.off = (const int*)d->split_off.p;
+  t.sp = (const int*)d->split_sp.p;
+  RVSP_HIP_TRY(launch_split_build(B, t, (int*)d->split_sp.p, s));
+  d->split = t;
+  d->split_valid = true;
+  return RVSP_SUCCESS;
+}
+
 rvsp_status_t run_numeric(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
                           const int* c_ci, float* c_val, hipStream_t s) {
   const RowStats st = row_stats(d);
+  for (int b = 1; b < d->numeric.table.rule.nbins; ++b)
+    if (d->numeric.table.spec[b].kind == KIND_BLOCK && bin_rows(d->numeric, b) > 0) {
+      RVSP_TRY(ensure_split(d, B, s));
+      break;
+    }
   for (int b = 1; b < d->numeric.table.rule.nbins; ++b) {
     const int n = bin_rows(d->numeric, b);
     if (n == 0) continue;
@@ -435,7 +476,8 @@ rvsp_status_t run_numeric(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B
                                           c_val, s));
     } else {
       RVSP_HIP_TRY(launch_numeric_bin(spec, rows, n, A, B, st, (const int*)d->c_rp.p, c_ci,
-                                      c_val, d->gs_dense, d->numeric.table.rule.bound[b], s));
+                                      c_val, d->gs_dense, d->numeric.table.rule.bound[b],
+                                      d->split, s));
     }
   }
   return RVSP_SUCCESS;
@@ -552,7 +594,7 @@ void rvsp_spgemm_descr_destroy(rvsp_spgemm_descr_t descr) {
                     &d->fill.rows, &d->numeric.rows, &d->c_ci_dev, &d->c_val_dev,
                     &d->gbits, &d->gdense, &d->learn_feat, &d->bin_stats,
                     &d->map, &d->poff, &d->map_weights, &d->keep, &d->koff,
-                    &d->keep_w};
+                    &d->keep_w, &d->split_cnt, &d->split_off, &d->split_sp};
   for (DevBuf* b : bufs) release(*b);
   destroy_graph(d);
   destroy_batch(d);
@@ -577,6 +619,7 @@ void rvsp_spgemm_invalidate_structure(rvsp_spgemm_descr_t descr) {
     descr->filled_into = NULL;
     descr->dev_cols_valid = false;
     descr->map_valid = false;
+    descr->split_valid = false;
     invalidate_batch(descr);
   }
 }

@@ -86,6 +86,12 @@ struct rvsp_spgemm_descr {
   bool map_promoted = false;  // searching bins already moved to the map
   bool position_map = true;   // promote on the first numeric-only compute
 
+  // Split table of B's long rows for the tiled numeric kernel's narrowing
+  // searches (per-row counts, offsets, table).
+  rvsp::DevBuf split_cnt, split_off, split_sp;
+  rvsp::SplitTable split;
+  bool split_valid = false;
+
   // Batched products (batch.cpp): stacked operands and one inner
   // descriptor per group of pairs; settings apply at the next estimation.
   rvsp::BatchState* batch = nullptr;
@@ -117,6 +123,7 @@ rvsp_status_t run_numeric(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B
 // storage); bins that do not fit the memory budget fall back to SEARCH.
 rvsp_status_t plan_map(rvsp_spgemm_descr* d, hipStream_t s);
 // Builds the map entries if they are not valid for the current structure.
+rvsp_status_t ensure_split(rvsp_spgemm_descr* d, const DevCsr& B, hipStream_t s);
 rvsp_status_t ensure_map_built(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
                                const int* c_ci, hipStream_t s);
 // Moves the non-empty searching numeric bins to the position map (chunked

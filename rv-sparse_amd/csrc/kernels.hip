@@ -1523,6 +1523,20 @@ __global__ __launch_bounds__(kBlock) void k_numeric_rowblock(
 // First index in [lo, hi] of the ascending b_ci[lo, hi) whose value is >= key,
 // searched forward from lo: doubling steps, then halving (cheap when the
 // answer is near lo).
+// gm_lower_bound over the B row [b0, be) through its split-table entries
+// `sp` (meta = sp[0..1]), for an answer known to be >= from.
+__device__ __forceinline__ int split_lower_bound(const int* __restrict__ b_ci,
+                                                 const int* __restrict__ sp, int2 meta, int b0,
+                                                 int be, int from, int key) {
+  const int g = meta.y & 31, nb = meta.y >> 5;
+  if (key <= meta.x) return max(b0, from);
+  const int j = (int)min((long long)nb, ((long long)key - meta.x) >> g);
+  // Granule j: positions [bound(j), bound(j + 1)), bound(0) = b0, bound(nb) = be.
+  const int s = j <= 0 ? b0 : (j >= nb ? be : sp[1 + j]);
+  const int e = j + 1 >= nb ? be : sp[2 + j];
+  return gm_lower_bound(b_ci, max(s, from), max(e, from), key);
+}
+
 __device__ __forceinline__ int gm_gallop(const int* __restrict__ b_ci, int lo, int hi,
                                          int key) {
   if (lo >= hi || b_ci[lo] >= key) return lo;
@@ -1610,7 +1624,7 @@ __device__ __forceinline__ int tiled_index_pos(const uint2* idx, int clo, int co
 __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
     const int* __restrict__ rows, int nrows, int T, DevCsr A, DevCsr B,
     const int* __restrict__ c_rp, const int* __restrict__ c_ci,
-    float* __restrict__ c_val) {
+    float* __restrict__ c_val, SplitTable split) {
   extern __shared__ __attribute__((aligned(16))) int tiled_lds_raw[];
   __shared__ int scan_tmp[kBlock / kWave];
   const TiledLds L = tiled_lds(tiled_lds_raw, T);
@@ -1683,16 +1697,28 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
           a = A.val[ap];
           int be = B.rp[k + 1];
           if (narrow) {
+            const int b0 = B.rp[k];
+            // A long B row's split table bounds both searches to one granule.
+            int so = -1;
+            if (split.off != nullptr && be - b0 >= kSplitMin && split.off[k + 1] > split.off[k])
+              so = split.off[k];
+            int2 meta = make_int2(0, 0);
+            if (so >= 0) meta = *reinterpret_cast<const int2*>(split.sp + so);
             int lo = 0;
             if (cursors && t > t_beg) {
 #pragma unroll
               for (int c = 0; c < kCurChunks; ++c)
                 if (c == chunk) lo = cur[c];
+            } else if (so >= 0) {
+              lo = split_lower_bound(B.ci, split.sp + so, meta, b0, be, b0, clo);
             } else {
-              lo = gm_lower_bound(B.ci, B.rp[k], be, clo);
+              lo = gm_lower_bound(B.ci, b0, be, clo);
             }
             bs = lo;
-            be = gm_gallop(B.ci, lo, be, chi + 1);
+            if (so >= 0)
+              be = split_lower_bound(B.ci, split.sp + so, meta, b0, be, lo, chi + 1);
+            else
+              be = gm_gallop(B.ci, lo, be, chi + 1);
             if (cursors) {
 #pragma unroll
               for (int c = 0; c < kCurChunks; ++c)
@@ -1874,7 +1900,8 @@ static hipError_t launch_numeric_g(const BinSpec& b, const int* rows, int nrows,
 hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
                               const DevCsr& A, const DevCsr& B, RowStats st,
                               const int* c_rp, const int* c_ci, float* c_val,
-                              const GlobalSlots& gs, int max_row, hipStream_t s) {
+                              const GlobalSlots& gs, int max_row, const SplitTable& split,
+                              hipStream_t s) {
   (void)st;
   if (nrows <= 0) return hipSuccess;
   switch (b.kind) {
@@ -1897,7 +1924,7 @@ hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
       const int ytiles = (int)std::min<long long>(16, (most + b.cap - 1) / b.cap);
       const int gx = (std::min(nrows, 1 << 20) + 7) / 8 * 8;
       hipLaunchKernelGGL(k_numeric_tiled, dim3(gx, ytiles), dim3(kBlock), b.lds_bytes, s,
-                         rows, nrows, b.cap, A, B, c_rp, c_ci, c_val);
+                         rows, nrows, b.cap, A, B, c_rp, c_ci, c_val, split);
       return hipGetLastError();
     }
     case KIND_GLOBAL:
@@ -1909,6 +1936,51 @@ hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
     default:
       return hipErrorInvalidValue;
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_split_counts(DevCsr B, int* __restrict__ cnt) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= B.rows) return;
+  const int len = B.rp[k + 1] - B.rp[k];
+  cnt[k] = len >= kSplitMin ? (2 + split_granules(len) + 1) & ~1 : 0;
+}
+
+hipError_t launch_split_counts(const DevCsr& B, int* cnt, hipStream_t s) {
+  if (B.rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_split_counts, dim3((B.rows + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                     B, cnt);
+  return hipGetLastError();
+}
+
+// One wave per row with a table: every boundary by its own binary search.
+__global__ __launch_bounds__(kBlock) void k_split_build(DevCsr B, SplitTable t,
+                                                        int* __restrict__ sp) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const int nwaves = gridDim.x * (kBlock / kWave);
+  for (int k = wave; k < B.rows; k += nwaves) {
+    const int so = t.off[k];
+    if (t.off[k + 1] == so) continue;
+    const int s = B.rp[k], e = B.rp[k + 1];
+    const int nb = split_granules(e - s);
+    const int lo = B.ci[s], hi = B.ci[e - 1];
+    int g = 0;
+    while (((hi - lo) >> g) >= nb) ++g;
+    if (lane == 0) {
+      sp[so] = lo;
+      sp[so + 1] = g | (nb << 5);
+    }
+    for (int j = 1 + lane; j <= nb; j += kWave) {
+      const long long key = (long long)lo + ((long long)j << g);
+      sp[so + 1 + j] = key > INT_MAX ? e : gm_lower_bound(B.ci, s, e, (int)key);
+    }
+  }
+}
+
+hipError_t launch_split_build(const DevCsr& B, const SplitTable& t, int* sp, hipStream_t s) {
+  if (B.rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_split_build, dim3(2048), dim3(kBlock), 0, s, B, t, sp);
+  return hipGetLastError();
 }
 
 // Fill of rows whose sorted columns the count phase kept: one workgroup per

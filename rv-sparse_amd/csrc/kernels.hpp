@@ -140,10 +140,33 @@ hipError_t launch_fill_bin(const BinSpec& b, const int* rows, int nrows,
 // --- numeric ------------------------------------------------------------
 // max_row: the largest C-row length the bin admits (its bound); KIND_BLOCK
 // sizes its tile grid from it (tiles of b.cap positions).
+// Split table of B's long rows (launch_split_*). A row k with at least
+// kSplitMin entries and columns lo..hi gets nb = clamp(len / kSplitPer, 8,
+// 256) granules of 2^g columns (the smallest g with (hi - lo) >> g < nb).
+// Its entries at sp[off[k]]: lo, g | nb << 5, then for j = 1..nb the first
+// position of B_k whose column is >= lo + (j << g) (padded to an even
+// count). Rows without a table have off[k] == off[k + 1]. KIND_BLOCK narrows
+// long B rows through it: a search then covers one granule.
+struct SplitTable {
+  const int* off = nullptr;  // B.rows + 1, or null: no table
+  const int* sp = nullptr;
+};
+constexpr int kSplitMin = 64;
+constexpr int kSplitPer = 8;
+__host__ __device__ inline int split_granules(int len) {
+  const int nb = len / kSplitPer;
+  return nb < 8 ? 8 : (nb > 256 ? 256 : nb);
+}
+// cnt (B.rows ints): table entries per row; scan it with launch_scan into
+// off, then launch_split_build fills sp.
+hipError_t launch_split_counts(const DevCsr& B, int* cnt, hipStream_t s);
+hipError_t launch_split_build(const DevCsr& B, const SplitTable& t, int* sp, hipStream_t s);
+
 hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
                               const DevCsr& A, const DevCsr& B, RowStats st,
                               const int* c_rp, const int* c_ci, float* c_val,
-                              const GlobalSlots& gs, int max_row, hipStream_t s);
+                              const GlobalSlots& gs, int max_row, const SplitTable& split,
+                              hipStream_t s);
 
 // Dynamic LDS of the tiled long-row numeric kernel for tiles of T positions.
 int tiled_lds_bytes(int T);

@@ -161,8 +161,10 @@ hipError_t launch_bin(rvsp_spgemm_descr* d, int phase, const BinSpec& spec, cons
         mv.width = d->map_width;
         return launch_numeric_map_bin(spec, mv, rows, n, A, B, (const int*)d->c_rp.p, c_val, s);
       }
+      if (spec.kind == KIND_BLOCK && ensure_split(d, B, s) != RVSP_SUCCESS)
+        return hipErrorOutOfMemory;
       return launch_numeric_bin(spec, rows, n, A, B, st, (const int*)d->c_rp.p, c_ci, c_val,
-                                d->gs_dense, max_row, s);
+                                d->gs_dense, max_row, d->split, s);
   }
 }
 
