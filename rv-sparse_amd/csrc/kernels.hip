@@ -1192,6 +1192,9 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
   int a_off = 0;          // subtracted from A positions to index staged metadata
   bool a_staged = false;  // uniform
   const T* rm;            // this row's map entries
+  bool rm_lds = false;    // uniform: rm points into the staged copy in LDS
+  int rm_shift = 0;       // bytes from mstage to the staged entry m0
+  long long rm_idx = 0;   // this row's first entry relative to m0
   if (seg) {
     // Rows first..last: their row pointers of C and A and their map offsets
     // are staged with coalesced loads, then the C segment's accumulators are
@@ -1252,6 +1255,9 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
     rm = staged ? reinterpret_cast<const T*>(reinterpret_cast<const char*>(mstage) + mshift) +
                       (mo - m0)
                 : map + mo;
+    rm_lds = staged;
+    rm_shift = mshift;
+    rm_idx = mo - m0;
   } else {
     const int row = active ? rows[r0 + g] : first;
     c_s = c_rp[row];
@@ -1282,65 +1288,78 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
     // Chunked loads: for up to kChunk A entries at once, their columns and
     // scales, then their B-row bounds, then the first kU products of each
     // are in flight together (three dependent rounds per chunk instead of
-    // one or more per entry); the entries are then applied in order.
+    // one or more per entry); the entries are then applied in order. A-entry
+    // loads past the row's last entry read a valid address (the last entry)
+    // and are masked, so every round issues at once. One lane's products of one
+    // entry have distinct positions, so their accumulators are read, added
+    // and written back together; entry u + 1's reads follow entry u's writes
+    // in program order.
     constexpr int kChunk = 5;
-    if (active) {
+    if (active && a_s < a_e) {
       float* rv = vals + base;
       int t0 = 0;
-      for (int c0 = a_s; c0 < a_e; c0 += kChunk) {
-        int bb[kChunk], ee[kChunk];
-        float aa[kChunk], vv[kChunk][kU];
-        // The chunk's A entries are split over the group's lanes (lane l
-        // loads entries l, l + G, ...) and shared by shuffles, so no entry
-        // is loaded twice.
-        constexpr int kPer = (kChunk + G - 1) / G;
-        int kl[kPer], bl[kPer], el[kPer];
-        float al[kPer];
+      auto apply_chunk = [&](const auto* mp) {
+        for (int c0 = a_s; c0 < a_e; c0 += kChunk) {
+          int bb[kChunk], ee[kChunk];
+          float aa[kChunk], vv[kChunk][kU];
+          constexpr int kPer = (kChunk + G - 1) / G;
+          int kl[kPer], bl[kPer], el[kPer];
+          float al[kPer];
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-          const int e = c0 + l + u * G;
-          const bool h = l + u * G < kChunk && e < a_e;
-          kl[u] = h ? A.ci[e] : 0;
-          al[u] = h ? A.val[e] : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-          const int e = c0 + l + u * G;
-          const bool h = l + u * G < kChunk && e < a_e;
-          bl[u] = h ? B.rp[kl[u]] : 0;
-          el[u] = h ? B.rp[kl[u] + 1] : 0;
-        }
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-          aa[u] = __shfl(al[u / G], u % G, G);
-          bb[u] = __shfl(bl[u / G], u % G, G);
-          ee[u] = __shfl(el[u / G], u % G, G);
-        }
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u)
-#pragma unroll
-          for (int j = 0; j < kU; ++j) {
-            const int p = bb[u] + l + j * G;
-            vv[u][j] = p < ee[u] ? B.val[p] : 0.0f;
+          for (int u = 0; u < kPer; ++u) {
+            const int e = min(c0 + l + u * G, a_e - 1);
+            kl[u] = A.ci[e];
+            al[u] = A.val[e];
           }
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-          const int len = ee[u] - bb[u];  // 0 past the row's last entry
+          for (int u = 0; u < kPer; ++u) {
+            bl[u] = B.rp[kl[u]];
+            el[u] = B.rp[kl[u] + 1];
+          }
 #pragma unroll
-          for (int j = 0; j < kU; ++j) {
-            const int t = l + j * G;
-            if (t < len) {
-              const int pos = rm[t0 + t];
-              rv[pos] = acc_step(rv[pos], aa[u], vv[u][j]);
+          for (int u = 0; u < kChunk; ++u) {
+            const bool h = c0 + u < a_e;
+            aa[u] = __shfl(al[u / G], u % G, G);
+            bb[u] = __shfl(bl[u / G], u % G, G);
+            ee[u] = h ? __shfl(el[u / G], u % G, G) : bb[u];
+          }
+#pragma unroll
+          for (int u = 0; u < kChunk; ++u)
+#pragma unroll
+            for (int j = 0; j < kU; ++j) {
+              const int p = bb[u] + l + j * G;
+              vv[u][j] = p < ee[u] ? B.val[p] : 0.0f;
             }
+#pragma unroll
+          for (int u = 0; u < kChunk; ++u) {
+            const int len = ee[u] - bb[u];  // 0 past the row's last entry
+            int pos[kU];
+            float cur[kU];
+#pragma unroll
+            for (int j = 0; j < kU; ++j) {
+              const int t = l + j * G;
+              pos[j] = t < len ? (int)mp[t0 + t] : -1;
+            }
+#pragma unroll
+            for (int j = 0; j < kU; ++j) cur[j] = pos[j] >= 0 ? rv[pos[j]] : 0.0f;
+#pragma unroll
+            for (int j = 0; j < kU; ++j)
+              if (pos[j] >= 0) rv[pos[j]] = acc_step(cur[j], aa[u], vv[u][j]);
+            for (int t = kU * G + l; t < len; t += G) {
+              const int q = mp[t0 + t];
+              rv[q] = acc_step(rv[q], aa[u], B.val[bb[u] + t]);
+            }
+            t0 += len;
           }
-          for (int t = kU * G + l; t < len; t += G) {
-            const int pos = rm[t0 + t];
-            rv[pos] = acc_step(rv[pos], aa[u], B.val[bb[u] + t]);
-          }
-          t0 += len;
         }
-      }
+      };
+      // Separate instantiations, so the staged map is read with LDS
+      // instructions and the unstaged one with global loads.
+      if (rm_lds)
+        apply_chunk(reinterpret_cast<const T*>(reinterpret_cast<const char*>(mstage) + rm_shift) +
+                    rm_idx);
+      else
+        apply_chunk(map + mo);
     }
   } else {
   // Load pipeline: entry ap's B values (kU per lane) in registers, entry
