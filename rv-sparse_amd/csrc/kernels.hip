@@ -1564,40 +1564,44 @@ __device__ __forceinline__ int gm_gallop(const int* __restrict__ b_ci, int lo, i
   return gm_lower_bound(b_ci, lo + (bound >> 1) + 1, min(lo + bound, hi), key);
 }
 
-// Long rows: one workgroup per (row, run of consecutive tiles of T output
-// positions). A tile's columns and accumulators live in LDS. The row's
-// products are streamed in reference order (A entries ascending, then B
-// positions ascending) in groups of kTiledItems products per lane: every lane
-// finds its products' A entries, gathers the B elements and looks their
-// columns up in the tile, all lanes in parallel. Only the accumulate is
-// ordered. Each wave owns every fourth 32-position stripe of the tile; a
-// group's products are routed to the owner's list in product order (a stable
-// split through LDS), and each wave applies its list 64 products at a time:
-// products of one batch that land on the same position are applied lowest
-// lane first (an LDS atomicMin on the position's tag picks it per round; one
-// wave's LDS operations execute in program order, so no barrier is needed).
-// Every C(i,j) thus sees exactly the reference's sequence of separately
-// rounded adds. A row spanning several tiles narrows every B row to each
-// tile's columns; along a workgroup's run of tiles the narrowed range of a B
-// row starts where the previous tile's ended (a per-lane cursor, for A rows
-// of up to kCurChunks * kBlock entries) and its end is found by galloping.
-struct TiledSlot {
-  int tag;    // lane of the lowest pending product of this round (INT_MAX: none)
-  float val;  // accumulator
-};
-
+// Long rows: one workgroup per (row, run of consecutive output positions).
+// The run is cut into tiles of at most T positions whose columns span fewer
+// than 32 * (T / 4) columns; a tile's accumulators live in LDS and its
+// positions are found through a bitmap index of its column span (per 32
+// columns the set bits and the positions before them). Where the span allows
+// fewer than T / 2 positions, the tile holds T / 2 positions instead, whose
+// columns replace the index and are binary-searched. The row's products
+// are streamed in reference order (A entries ascending, then B positions
+// ascending) in groups of kTiledItems products per lane: every lane finds its
+// products' A entries, gathers the B elements and looks their columns up in
+// the tile, all lanes in parallel. Only the accumulate is ordered. Each wave
+// owns every fourth 32-position stripe of the tile; a group's products are
+// routed to the owner's list in product order (a stable split through LDS),
+// and each wave applies its list 64 products at a time: products of one batch
+// that land on the same position are applied lowest lane first (per round an
+// LDS atomicMin of (position, lane) on a small per-wave hash of positions
+// picks, for every position, its lowest pending lane; one wave's LDS
+// operations execute in program order, so no barrier is needed). Every C(i,j)
+// thus sees exactly the reference's sequence of separately rounded adds. A
+// tile that is not the whole row narrows every B row to the tile's columns;
+// along a workgroup's run the narrowed range of a B row starts where the
+// previous tile's ended (a per-lane cursor, for A rows of up to kCurChunks *
+// kBlock entries), and split tables bound the searches of long B rows.
 constexpr int kTiledItems = 2;             // products per lane per group
 constexpr int kTiledGroup = kTiledItems * kBlock;
 constexpr int kTiledWaves = kBlock / kWave;
+constexpr int kTagSlots = 256;             // per-wave hash of pending positions
+constexpr int kEntrySlots = 1024;          // entry table of a chunk
+constexpr int kPrePad = 8;                 // entries probed past a table hit
 
-constexpr int kEntrySlots = 1024;  // entry table of a chunk (see k_numeric_tiled)
-constexpr int kPrePad = 8;         // entries probed past a table hit
+__host__ __device__ constexpr int tiled_index_words(int T) { return T / 4 > 64 ? T / 4 : 64; }
 
 struct TiledLds {
-  int* cnt;         // [item][wave][owner] products of a group
-  TiledSlot* slot;  // T
+  int* cnt;         // [owner][item][wave] products of a group
+  float* vals;      // T accumulators
   int2* stage;      // kTiledGroup (position, product bits), grouped by owner wave
-  int* cols;        // T: the tile's columns, or its bitmap index (uint2 per 32 columns)
+  uint2* idx;       // tiled_index_words(T) words of the tile's bitmap index
+  int* tags;        // kTiledWaves x kTagSlots
   int* etab;        // kEntrySlots: entry of product m << shift
   int* pre;         // kBlock + kPrePad exclusive prefix of the chunk's product counts
   int* bstart;      // kBlock B-row starts
@@ -1606,11 +1610,12 @@ struct TiledLds {
 
 __device__ __forceinline__ TiledLds tiled_lds(int* lds, int T) {
   TiledLds L;
-  L.cnt = lds;  // first: 16-byte aligned for int4 reads; slot and stage stay 8-byte aligned for any T
-  L.slot = reinterpret_cast<TiledSlot*>(lds + kTiledItems * kTiledWaves * kTiledWaves);
-  L.stage = reinterpret_cast<int2*>(L.slot + T);
-  L.cols = reinterpret_cast<int*>(L.stage + kTiledGroup);
-  L.etab = L.cols + T;
+  L.cnt = lds;  // 16-byte aligned; every later array stays 8-byte aligned
+  L.stage = reinterpret_cast<int2*>(lds + kTiledItems * kTiledWaves * kTiledWaves);
+  L.idx = reinterpret_cast<uint2*>(L.stage + kTiledGroup);
+  L.vals = reinterpret_cast<float*>(L.idx + tiled_index_words(T));
+  L.tags = reinterpret_cast<int*>(L.vals + ((T + 1) & ~1));
+  L.etab = L.tags + kTiledWaves * kTagSlots;
   L.pre = L.etab + kEntrySlots;
   L.bstart = L.pre + kBlock + kPrePad;
   L.aval = reinterpret_cast<float*>(L.bstart + kBlock);
@@ -1632,8 +1637,7 @@ __device__ __forceinline__ int tiled_entry(const TiledLds& L, int shift, int q) 
   return e;
 }
 
-// Position of column col (clo <= col <= chi, present) in the tile, from the
-// bitmap index: per 32 columns the set bits and the positions before them.
+// Position of column col (clo <= col <= chi, present) in the tile.
 __device__ __forceinline__ int tiled_index_pos(const uint2* idx, int clo, int col) {
   const int d = col - clo;
   const uint2 v = idx[d >> 5];
@@ -1647,61 +1651,78 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
   extern __shared__ __attribute__((aligned(16))) int tiled_lds_raw[];
   __shared__ int scan_tmp[kBlock / kWave];
   const TiledLds L = tiled_lds(tiled_lds_raw, T);
+  const int span = 32 * tiled_index_words(T);  // columns a tile may span
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wid = tid >> 6;
   const unsigned long long below = (1ull << lane) - 1ull;
+  int* my_tags = L.tags + wid * kTagSlots;
   if (tid < kPrePad) L.pre[kBlock + tid] = INT_MAX;  // probes past the chunk end
+  for (int i = tid; i < kTiledWaves * kTagSlots; i += kBlock) L.tags[i] = INT_MAX;
+  __syncthreads();
   for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
     const int row = rows[r];
     const int c_s = c_rp[row];
     const int n = c_rp[row + 1] - c_s;
-    const int ntiles = (n + T - 1) / T;
-    // This workgroup's run of tiles.
-    const int per = (ntiles + gridDim.y - 1) / gridDim.y;
-    const int t_beg = blockIdx.y * per;
-    const int t_end = min(ntiles, t_beg + per);
-    const bool narrow = ntiles > 1;
+    // This workgroup's run: one of up to gridDim.y equal parts of the row.
+    const int runs = (int)min((long long)gridDim.y, ((long long)n + T - 1) / T);
+    if ((int)blockIdx.y >= runs) continue;
+    const int P0 = (int)((long long)n * blockIdx.y / runs);
+    const int P1 = (int)((long long)n * (blockIdx.y + 1) / runs);
     const int a_s = A.rp[row], a_e = A.rp[row + 1];
     const bool cursors = a_e - a_s <= kCurChunks * kBlock;
     int cur[kCurChunks];  // per chunk: where this lane's B row continues
-    for (int t = t_beg; t < t_end; ++t) {
-      const int t0 = t * T;
-      const int tn = min(T, n - t0);
-      const int clo = c_ci[c_s + t0];
-      const int chi = c_ci[c_s + t0 + tn - 1];
-      // Position lookup: a bitmap index over the tile's column span when it
-      // fits the column area (two words per 32 columns), else binary search.
+    bool first = true;
+    for (int p0 = P0; p0 < P1;) {
+      const int clo = c_ci[c_s + p0];
+      // The tile: up to T positions whose columns stay within the span of
+      // the index, or, where that holds fewer positions, up to T / 2
+      // positions whose columns are kept in the index's place and searched.
+      int pe = min(P1, p0 + T);
+      if (c_ci[c_s + pe - 1] - clo >= span) {
+        // First position at or past clo + span: a probe every `step`
+        // positions, then every position of the step that holds it.
+        const int len = pe - p0;
+        const int step = (len + kBlock - 1) / kBlock;
+        const int c1 = __syncthreads_count(c_ci[c_s + p0 + min(len - 1, tid * step)] - clo < span);
+        const int b1 = p0 + (c1 - 1) * step;  // last probe inside the span
+        const int q2 = b1 + 1 + tid;
+        const int c2 = __syncthreads_count(tid < step - 1 && q2 < p0 + len &&
+                                           c_ci[c_s + q2] - clo < span);
+        pe = b1 + 1 + c2;
+      }
+      const bool use_idx = pe - p0 >= min(P1 - p0, T / 2);  // uniform
+      if (!use_idx) pe = min(P1, p0 + T / 2);
+      const int tn = pe - p0;
+      const int chi = c_ci[c_s + pe - 1];
+      const bool narrow = !(p0 == 0 && pe == n);
+      int* cols = reinterpret_cast<int*>(L.idx);  // T / 2 columns in cols mode
       const int iw = ((chi - clo) >> 5) + 1;
-      const bool use_idx = iw <= T / 2;
-      uint2* idx = reinterpret_cast<uint2*>(L.cols);
-      if (use_idx) {
-        for (int w = tid; w < iw; w += kBlock) idx[w] = make_uint2(0u, 0u);
-      }
-      for (int j = tid; j < tn; j += kBlock) {
-        L.slot[j].tag = INT_MAX;
-        L.slot[j].val = 0.0f;
-      }
+      if (use_idx)
+        for (int w = tid; w < iw; w += kBlock) L.idx[w] = make_uint2(0u, 0u);
+      for (int j = tid; j < tn; j += kBlock) L.vals[j] = 0.0f;
       __syncthreads();
       if (use_idx) {
         for (int j = tid; j < tn; j += kBlock) {
-          const int d = c_ci[c_s + t0 + j] - clo;
-          atomicOr(&idx[d >> 5].x, 1u << (d & 31));
+          const int d = c_ci[c_s + p0 + j] - clo;
+          atomicOr(&L.idx[d >> 5].x, 1u << (d & 31));
         }
-        __syncthreads();
+      } else {
+        for (int j = tid; j < tn; j += kBlock) cols[j] = c_ci[c_s + p0 + j];
+      }
+      __syncthreads();
+      if (use_idx) {
         // Positions before each word: per thread a run of consecutive words.
         const int per_t = (iw + kBlock - 1) / kBlock;
         const int w_beg = min(iw, tid * per_t), w_end = min(iw, w_beg + per_t);
         int cnt_t = 0;
-        for (int w = w_beg; w < w_end; ++w) cnt_t += __popc(idx[w].x);
+        for (int w = w_beg; w < w_end; ++w) cnt_t += __popc(L.idx[w].x);
         int tot_w;
         int run = block_excl_scan(cnt_t, scan_tmp, &tot_w);
         for (int w = w_beg; w < w_end; ++w) {
-          idx[w].y = (unsigned)run;
-          run += __popc(idx[w].x);
+          L.idx[w].y = (unsigned)run;
+          run += __popc(L.idx[w].x);
         }
-      } else {
-        for (int j = tid; j < tn; j += kBlock) L.cols[j] = c_ci[c_s + t0 + j];
       }
       __syncthreads();
       int chunk = 0;
@@ -1724,7 +1745,7 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
             int2 meta = make_int2(0, 0);
             if (so >= 0) meta = *reinterpret_cast<const int2*>(split.sp + so);
             int lo = 0;
-            if (cursors && t > t_beg) {
+            if (cursors && !first) {
 #pragma unroll
               for (int c = 0; c < kCurChunks; ++c)
                 if (c == chunk) lo = cur[c];
@@ -1811,7 +1832,7 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
             prod[i] = 0.0f;
             own[i] = kTiledWaves;  // no product
             if (col[i] >= clo && col[i] <= chi) {  // col -1: no product
-              pos[i] = use_idx ? tiled_index_pos(idx, clo, col[i]) : lds_find(L.cols, tn, col[i]);
+              pos[i] = use_idx ? tiled_index_pos(L.idx, clo, col[i]) : lds_find(cols, tn, col[i]);
               prod[i] = av[i] * bv[i];  // rounded here, added below
               own[i] = (pos[i] >> 5) & (kTiledWaves - 1);
             }
@@ -1863,39 +1884,42 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
           for (int k0 = 0; k0 < my_n; k0 += kWave) {
             const int j = k0 + lane;
             bool act = j < my_n;
-            int p = 0;
+            int p = 0, key = 0, slot = 0;
             float v = 0.0f;
             if (act) {
               const int2 sv = L.stage[my_base + j];
               p = sv.x;
               v = __int_as_float(sv.y);
+              key = (p << 6) | lane;
+              slot = (int)(((unsigned)p * 2654435761u) >> 24);  // kTagSlots = 256
             }
+            // Per round, each hash slot keeps its smallest (position, lane):
+            // for every position the lowest lane still pending is the only
+            // one that can win, so its products are applied in lane order.
             while (__any(act)) {
-              if (act) atomicMin(&L.slot[p].tag, lane);
-              if (act) {
-                const TiledSlot cur_s = L.slot[p];
-                if (cur_s.tag == lane) {
-                  TiledSlot w;
-                  w.tag = INT_MAX;
-                  w.val = cur_s.val + v;
-                  L.slot[p] = w;
-                  act = false;
-                }
+              if (act) atomicMin(&my_tags[slot], key);
+              if (act && my_tags[slot] == key) {
+                L.vals[p] = L.vals[p] + v;
+                my_tags[slot] = INT_MAX;
+                act = false;
               }
             }
           }
         }
         __syncthreads();  // the chunk's metadata is replaced next
       }
-      for (int j = tid; j < tn; j += kBlock) c_val[c_s + t0 + j] = L.slot[j].val;
+      for (int j = tid; j < tn; j += kBlock) c_val[c_s + p0 + j] = L.vals[j];
       __syncthreads();  // the tile's LDS is reused by the next tile
+      p0 = pe;
+      first = false;
     }
   }
 }
 
 int tiled_lds_bytes(int T) {
-  return (3 * T + 2 * kTiledGroup + kTiledItems * kTiledWaves * kTiledWaves +
-          kEntrySlots + 3 * kBlock + kPrePad) * 4;
+  return kTiledItems * kTiledWaves * kTiledWaves * 4 + kTiledGroup * 8 +
+         tiled_index_words(T) * 8 + ((T + 1) & ~1) * 4 + kTiledWaves * kTagSlots * 4 +
+         (kEntrySlots + 3 * kBlock + kPrePad) * 4;
 }
 
 template <int G>

@@ -171,7 +171,7 @@ hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
 // Dynamic LDS of the tiled long-row numeric kernel for tiles of T positions.
 int tiled_lds_bytes(int T);
 // Default tile of the long-row numeric kernel: three workgroups fit a CU.
-constexpr int kTiledPositions = 3072;
+constexpr int kTiledPositions = 6144;
 
 // --- position map (numeric structure cache) -------------------------------
 struct MapView {
