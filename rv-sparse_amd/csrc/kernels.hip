@@ -1665,10 +1665,13 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
     const int c_s = c_rp[row];
     const int n = c_rp[row + 1] - c_s;
     // This workgroup's run: one of up to gridDim.y equal parts of the row.
-    const int runs = (int)min((long long)gridDim.y, ((long long)n + T - 1) / T);
+    // (32-bit arithmetic only: 64-bit division expands into fused float
+    // operations, which no device code here may contain.)
+    const int runs = min((int)gridDim.y, n / T + (n % T != 0));
     if ((int)blockIdx.y >= runs) continue;
-    const int P0 = (int)((long long)n * blockIdx.y / runs);
-    const int P1 = (int)((long long)n * (blockIdx.y + 1) / runs);
+    const int rq = n / runs, rr = n % runs, y = blockIdx.y;
+    const int P0 = rq * y + rr * y / runs;                 // floor(n * y / runs)
+    const int P1 = rq * (y + 1) + rr * (y + 1) / runs;
     const int a_s = A.rp[row], a_e = A.rp[row + 1];
     const bool cursors = a_e - a_s <= kCurChunks * kBlock;
     int cur[kCurChunks];  // per chunk: where this lane's B row continues
