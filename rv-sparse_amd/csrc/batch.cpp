@@ -425,6 +425,7 @@ rvsp_status_t rvsp_spgemm_batch_work_estimation(rvsp_spgemm_descr_t descr, int32
     g.r0 = bs.a_row_off[g.p0];
     g.r1 = bs.a_row_off[g.p1];
     RVSP_TRY(rvsp_spgemm_descr_create(&g.inner));
+    if (&g != &bs.groups.front()) g.inner->split_owner = bs.groups.front().inner;
     RVSP_TRY(rvsp_spgemm_set_algo(g.inner, d->algo));
     RVSP_TRY(rvsp_spgemm_set_memory(g.inner, RVSP_MEMORY_DEVICE));
     RVSP_TRY(rvsp_spgemm_set_stream(g.inner, (void*)s));

@@ -420,6 +420,12 @@ rvsp_status_t run_fill(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
 
 rvsp_status_t ensure_split(rvsp_spgemm_descr* d, const DevCsr& B, hipStream_t s) {
   if (d->split_valid) return RVSP_SUCCESS;
+  if (d->split_owner) {
+    RVSP_TRY(ensure_split(d->split_owner, B, s));
+    d->split = d->split_owner->split;
+    d->split_valid = d->split_owner->split_valid;
+    return RVSP_SUCCESS;
+  }
   d->split = SplitTable();
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   RVSP_HIP_TRY(hipStreamIsCapturing(s, &cap));

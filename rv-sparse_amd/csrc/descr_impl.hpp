@@ -91,6 +91,9 @@ struct rvsp_spgemm_descr {
   rvsp::DevBuf split_cnt, split_off, split_sp;
   rvsp::SplitTable split;
   bool split_valid = false;
+  // A descriptor over the same B whose table this one uses (the groups of a
+  // batch share one stacked B); not owned.
+  rvsp_spgemm_descr* split_owner = nullptr;
 
   // Batched products (batch.cpp): stacked operands and one inner
   // descriptor per group of pairs; settings apply at the next estimation.
