@@ -81,16 +81,20 @@ __device__ __forceinline__ int advance_entry(const int* pre, int j, int p) {
 }
 
 // Whole-workgroup version of group_for_each_col (blockDim.x == NT).
-// `sh_excl` and `sh_bs` hold NT ints each, `tmp` NT / kWave ints.
-// Must be reached by every thread of the block.
+// `sh_excl` holds NT + kBlockPad ints, `sh_bs` and `etab` NT ints each
+// (etab: the entry of every (1 << shift)-th product) and `tmp` NT / kWave
+// ints. Must be reached by every thread of the block.
+constexpr int kBlockPad = 8;      // prefixes probed past an entry-table hit
+constexpr int kBlockStreams = 4;  // products in flight per lane
 template <int NT, class F>
 __device__ __forceinline__ void block_for_each_col(int a_s, int a_e,
                                                    const int* __restrict__ a_ci,
                                                    const int* __restrict__ b_rp,
                                                    const int* __restrict__ b_ci,
-                                                   int* sh_excl, int* sh_bs,
+                                                   int* sh_excl, int* sh_bs, int* etab,
                                                    int* tmp, F&& f) {
   const int t = threadIdx.x;
+  if (t < kBlockPad) sh_excl[NT + t] = INT_MAX;  // probes past the chunk end
   for (int base = a_s; base < a_e; base += NT) {
     const int ap = base + t;
     int bs = 0, len = 0;
@@ -103,27 +107,34 @@ __device__ __forceinline__ void block_for_each_col(int a_s, int a_e,
     const int excl = block_excl_scan<int, NT>(len, tmp, &total);
     sh_excl[t] = excl;
     sh_bs[t] = bs;
+    // Entry table: slot m holds the entry of product m << shift.
+    int shift = 5;
+    while ((total >> shift) >= NT) ++shift;
+    for (int m = (excl + (1 << shift) - 1) >> shift; (m << shift) < excl + len; ++m) etab[m] = t;
     __syncthreads();
-    // Two interleaved streams per lane (products t + 2k NT and
-    // t + (2k+1) NT), so two gathers are in flight; each stream keeps the
-    // entry of its last product and gallops forward from it.
-    // The loop is uniform over the block (f sees every lane, with a flag for
-    // lanes past the end), so f may use cross-lane operations.
-    int ja = 0, jb = 0;
-    for (int p0 = 0; p0 < total; p0 += 2 * NT) {
-      const int p = p0 + t;
-      const int pb = p + NT;
-      int ca = -1, cb = -1;
-      if (p < total) {
-        ja = advance_entry<NT>(sh_excl, ja, p);
-        ca = b_ci[sh_bs[ja] + (p - sh_excl[ja])];
+    // kBlockStreams interleaved streams per lane (products t + (S k + s) NT),
+    // each product's entry found independently (table slot, then the next
+    // kBlockPad prefixes at once), so all their gathers are in flight
+    // together. The loop is uniform over the block (f sees every lane, with
+    // a flag for lanes past the end), so f may use cross-lane operations.
+    for (int p0 = 0; p0 < total; p0 += kBlockStreams * NT) {
+      int c[kBlockStreams];
+#pragma unroll
+      for (int st = 0; st < kBlockStreams; ++st) {
+        const int p = p0 + st * NT + t;
+        c[st] = -1;
+        if (p < total) {
+          int e = etab[p >> shift];
+          int n = 0;
+#pragma unroll
+          for (int j = 1; j <= kBlockPad; ++j) n += sh_excl[e + j] <= p ? 1 : 0;
+          e += n;
+          if (n == kBlockPad) e = advance_entry<NT>(sh_excl, e, p);
+          c[st] = b_ci[sh_bs[e] + (p - sh_excl[e])];
+        }
       }
-      if (pb < total) {
-        jb = advance_entry<NT>(sh_excl, max(ja, jb), pb);
-        cb = b_ci[sh_bs[jb] + (pb - sh_excl[jb])];
-      }
-      f(ca, p < total);
-      f(cb, pb < total);
+#pragma unroll
+      for (int st = 0; st < kBlockStreams; ++st) f(c[st], p0 + st * NT + t < total);
     }
     __syncthreads();
   }
@@ -498,7 +509,7 @@ __global__ __launch_bounds__(NT) void k_bitmap_row(
   constexpr bool FILL = MODE == kBitmapFill;
   constexpr int kWaves = NT / kWave;
   extern __shared__ unsigned lbits[];
-  __shared__ int sh_excl[NT], sh_bs[NT], tmp[kWaves];
+  __shared__ int sh_excl[NT + kBlockPad], sh_bs[NT], sh_etab[NT], tmp[kWaves];
   unsigned* bits = GLOBAL_BM ? gbits + (size_t)blockIdx.x * slot_words : lbits;
   for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
     const int row = rows[r];
@@ -514,7 +525,7 @@ __global__ __launch_bounds__(NT) void k_bitmap_row(
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x >> 6;
     block_for_each_col<NT>(A.rp[row], A.rp[row + 1], A.ci, B.rp, B.ci, sh_excl,
-                       sh_bs, tmp, [&](int col, bool valid) {
+                       sh_bs, sh_etab, tmp, [&](int col, bool valid) {
                          if (valid) {
                            const int d = col - base_col;
                            __hip_atomic_fetch_or(&bits[d >> 5], 1u << (d & 31),
@@ -2036,12 +2047,15 @@ __global__ __launch_bounds__(kBlock) void k_copy_kept(const int* __restrict__ ro
                                                       const long long* __restrict__ koff,
                                                       const int* __restrict__ c_rp,
                                                       int* __restrict__ c_ci) {
+  // Kept rows are long: gridDim.y workgroups share each row.
   for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
     const int row = rows[r];
     const int c_s = c_rp[row];
     const int n = c_rp[row + 1] - c_s;
     const int* src = keep + koff[row];
-    for (int j = threadIdx.x; j < n; j += kBlock) c_ci[c_s + j] = src[j];
+    const int per = (n + gridDim.y - 1) / gridDim.y;
+    const int j0 = blockIdx.y * per, j1 = min(n, j0 + per);
+    for (int j = j0 + threadIdx.x; j < j1; j += kBlock) c_ci[c_s + j] = src[j];
   }
 }
 
@@ -2049,7 +2063,7 @@ hipError_t launch_copy_kept(const int* rows, int nrows, const int* keep,
                             const long long* koff, const int* c_rp, int* c_ci,
                             hipStream_t s) {
   if (nrows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_copy_kept, dim3(row_grid(nrows)), dim3(kBlock), 0, s, rows, nrows,
+  hipLaunchKernelGGL(k_copy_kept, dim3(row_grid(nrows), 8), dim3(kBlock), 0, s, rows, nrows,
                      keep, koff, c_rp, c_ci);
   return hipGetLastError();
 }

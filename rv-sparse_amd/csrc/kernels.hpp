@@ -199,8 +199,8 @@ hipError_t init_kernel_attributes(int lds_bytes);
 // Largest dynamic LDS a workgroup may request on gfx950.
 constexpr int kMaxLdsBytes = 160 * 1024;
 // LDS kept back from plans for the kernels' static arrays (the largest is the
-// 1024-thread bitmap kernel's, about 8.3 KB).
-constexpr int kStaticLdsReserve = 12 * 1024;
+// 1024-thread bitmap kernel's, about 12.3 KB).
+constexpr int kStaticLdsReserve = 16 * 1024;
 
 // A entries per row the SEARCH kernel's segment mode stages in LDS (a
 // workgroup whose rows hold more A entries reads them from global memory).

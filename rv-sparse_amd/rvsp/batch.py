@@ -128,10 +128,10 @@ def upload_pairs(pairs):
 
 
 def spgemm_batch(pairs, algo=F.ALGO_DEFAULT, contiguous=True, group_limit=None, graph=True,
-                 repeats=1):
+                 repeats=1, choices=None):
     """Host pairs in, host products out, through the batched device path."""
     As, Bs, owned = upload_pairs(pairs)
-    bt = Batch(algo=algo, group_limit=group_limit, graph=graph)
+    bt = Batch(algo=algo, group_limit=group_limit, graph=graph, choices=choices)
     out = None
     try:
         nnz = bt.estimate(As, Bs)

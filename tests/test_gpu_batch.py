@@ -68,6 +68,31 @@ def test_batch_of_row_blocks_shares_b(gpu):
         assert g.values.tobytes() == v[s:e].tobytes()
 
 
+def test_row_blocks_in_groups_share_the_split_table(gpu):
+    """Row blocks of C = A*A in several groups, every long row through the
+    tiled kernel with small tiles (many tiles per row, narrowed B rows): the
+    groups share one split table of the stacked B and every block still
+    equals the matching rows of the product."""
+    A = gen.rmat(12, seed=5)
+    cuts = [0, 300, 1100, 2600, A.rows]
+    blocks = []
+    for r0, r1 in zip(cuts[:-1], cuts[1:]):
+        rp = A.row_ptr[r0:r1 + 1] - A.row_ptr[r0]
+        s, e = A.row_ptr[r0], A.row_ptr[r1]
+        blocks.append(Csr(r1 - r0, A.cols, rp, A.col_idx[s:e], A.values[s:e]))
+    pairs = [(blk, A) for blk in blocks]
+    ops = [int(np.diff(A.row_ptr)[blk.col_idx].sum()) for blk in blocks]
+    choices = [(2, b, F.KIND_BLOCK, 256, 256) for b in range(2, 13)]
+    got, groups = spgemm_batch(pairs, group_limit=max(ops), choices=choices)
+    assert groups >= 3
+    rp, ci, v = oracle.spgemm(A, A)
+    for (r0, r1), g in zip(zip(cuts[:-1], cuts[1:]), got):
+        s, e = rp[r0], rp[r1]
+        assert np.array_equal(g.row_ptr, rp[r0:r1 + 1] - s)
+        assert g.col_idx.tobytes() == ci[s:e].tobytes()
+        assert g.values.tobytes() == v[s:e].tobytes()
+
+
 @pytest.mark.parametrize("graph", [True, False])
 def test_repeated_numeric_computes_with_new_values(gpu, graph):
     pairs = small_pairs(count=12, seed=2)
