@@ -75,8 +75,9 @@ typedef enum
     RVSP_KIND_HASH = 2,    /* LDS hash table, sub-wave group per row */
     RVSP_KIND_SEARCH = 3,  /* LDS sorted columns + binary search */
     RVSP_KIND_BITMAP = 4,  /* LDS bitmap over the row's column span */
-    RVSP_KIND_BLOCK = 5,   /* long rows: workgroup per tile of `cap` output
-                              positions, LDS columns and accumulators */
+    RVSP_KIND_BLOCK = 5,   /* long rows: workgroups over runs of output
+                              positions, tiles of up to `cap` positions with
+                              LDS accumulators and a column index */
     RVSP_KIND_GLOBAL = 6,  /* workgroup per row, dense global scratch */
     RVSP_KIND_MAP = 7      /* numeric via a cached position per product: the
                               descriptor records, on the first compute, where

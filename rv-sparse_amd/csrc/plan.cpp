@@ -81,8 +81,9 @@ int lds_bytes_for(int phase, int kind, int group, int cap, int variant) {
       bytes = cap / 8;
       break;
     case KIND_BLOCK:
-      // Tiles of `cap` positions: columns, accumulators and order tags,
-      // plus one chunk of A-entry metadata.
+      // Tiles of up to `cap` positions: accumulators, the column index,
+      // staging of one group of products, per-wave order tags and one chunk
+      // of A-entry metadata.
       if (phase != PHASE_NUMERIC || cap < kBlock) break;
       bytes = tiled_lds_bytes(cap);
       break;
