@@ -548,17 +548,23 @@ rvsp_status_t ensure_map_built(rvsp_spgemm_descr* d, const DevCsr& A, const DevC
   return RVSP_SUCCESS;
 }
 
+constexpr int kMapFromTiledMax = 2048;  // longest C row a tiled bin may move to the map with
+
 rvsp_status_t promote_to_map(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
                              const int* c_ci, hipStream_t s) {
   d->map_promoted = true;
   PhaseTable& t = d->numeric.table;
   bool any = false;
   for (int b = 2; b < t.rule.nbins; ++b) {
-    if (bin_rows(d->numeric, b) == 0 || t.spec[b].kind != KIND_SEARCH) continue;
+    if (bin_rows(d->numeric, b) == 0) continue;
+    // Searching bins keep their shape; tiled bins of rows short enough for
+    // a workgroup-sized accumulator take the widest map shape.
+    const bool block = t.spec[b].kind == KIND_BLOCK && t.rule.bound[b] <= kMapFromTiledMax;
+    if (t.spec[b].kind != KIND_SEARCH && !block) continue;
     BinChoice c;
     c.kind = KIND_MAP;
-    c.group = t.spec[b].group;
-    c.cap = t.spec[b].cap;
+    c.group = block ? 64 : t.spec[b].group;
+    c.cap = block ? t.rule.bound[b] : t.spec[b].cap;
     c.variant = 2;
     PhaseTable trial = t;  // a refused choice would turn the bin GLOBAL
     if (apply_choice(trial, PHASE_NUMERIC, b, c, t.rule.bound[b])) {

@@ -145,6 +145,39 @@ def test_position_map_repeated_numeric(gpu, name):
         dB.free()
 
 
+@pytest.mark.parametrize("name", ["cfg3_rmat_s12", "long_rows"])
+def test_tiled_bins_move_to_the_map_on_reuse(gpu, name):
+    # Tiled numeric bins of rows up to 2048 long move to the position map on
+    # the first numeric-only compute; every compute keeps the oracle's bits.
+    A, B = cases.build(name)
+    choices = [(2, b, F.KIND_BLOCK, 256, 1024) for b in range(2, 10)]
+    dA = rvsp.DeviceCsr.upload(A)
+    dB = dA if B is A else rvsp.DeviceCsr.upload(B)
+    d = rvsp.Descriptor(memory=F.MEMORY_DEVICE, choices=choices)
+    nnz = d.estimate(dA, dB)
+    before = [p for p in d.plan() if p["phase"] == 2 and p["rows"] > 0 and 2 <= p["bin"] < 10]
+    assert before and all(p["kind"] == "lds_block" for p in before)
+    dC = rvsp.DeviceCsr(A.rows, B.cols, nnz)
+    Cm = dC.struct(nnz)
+    rng = np.random.Generator(np.random.PCG64(5))
+    square = B is A
+    for it in range(3):
+        if it:
+            A = A.with_values(rng.uniform(-1, 1, A.nnz).astype(np.float32))
+            dA.set_values(A.values)
+            if square:
+                B = A
+        F.check(d.compute(dA, dB, Cm))
+        same(dC.download(), oracle.spgemm(A, B))
+    after = [p for p in d.plan() if p["phase"] == 2 and p["rows"] > 0 and 2 <= p["bin"] < 10]
+    assert all(p["kind"] == "lds_map" for p in after)
+    d.close()
+    dC.free()
+    dA.free()
+    if dB is not dA:
+        dB.free()
+
+
 @pytest.mark.parametrize("variant", sorted(PLAN_VARIANTS))
 @pytest.mark.parametrize("name", ["cfg4_pair_0", "long_rows", "rect_empty_rows",
                                   "cfg2_laplacian_g64_stencil", "cfg3_rmat_s12"])
