@@ -69,6 +69,25 @@ rvsp_status_t rvsp_spgemm_hip_f32(int32_t a_rows, int32_t a_cols,
                                   float **c_values_out, int32_t *c_nnz_out,
                                   int64_t *op_counts_out);
 
+/*
+ * The reference's scalar raw kernel (src/kernels/spgemm/rvsp_spgemm.h:79),
+ * which its bench harness calls (bench/bench.c:166): here the same GPU path
+ * as rvsp_spgemm_hip_f32, as the scalar backend of rvsp_spgemm_csr is.
+ */
+rvsp_status_t rvsp_spgemm_scalar_f32(int32_t a_rows, int32_t a_cols,
+                                     int32_t b_cols,
+                                     const int32_t *a_row_ptr,
+                                     const int32_t *a_col_idx,
+                                     const float *a_values,
+                                     const int32_t *b_row_ptr,
+                                     const int32_t *b_col_idx,
+                                     const float *b_values,
+                                     void *workspace, size_t workspace_bytes,
+                                     int32_t **c_row_ptr_out,
+                                     int32_t **c_col_idx_out,
+                                     float **c_values_out, int32_t *c_nnz_out,
+                                     int64_t *op_counts_out);
+
 #ifdef __cplusplus
 }
 #endif

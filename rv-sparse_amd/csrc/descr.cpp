@@ -852,7 +852,9 @@ rvsp_status_t rvsp_spgemm_csr(const rvsp_csr_matrix_t* A, const rvsp_csr_matrix_
 
 // Raw one-shot kernel with the reference's RVSP_SPGEMM_PARAMS
 // (gustavson_core_f32.inc:66-194): argument checks, then symbolic count,
-// C allocation, fill and numeric on the GPU.
+// C allocation, fill and numeric on the GPU. rvsp_spgemm_scalar_f32 (the
+// reference's scalar instance, rvsp_spgemm.h:79) is served by it, as the
+// scalar backend is.
 rvsp_status_t rvsp_spgemm_hip_f32(int32_t a_rows, int32_t a_cols, int32_t b_cols,
                                   const int32_t* a_row_ptr, const int32_t* a_col_idx,
                                   const float* a_values, const int32_t* b_row_ptr,
@@ -915,6 +917,19 @@ rvsp_status_t rvsp_spgemm_hip_f32(int32_t a_rows, int32_t a_cols, int32_t b_cols
   *c_values_out = val;
   *c_nnz_out = c_nnz;
   return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_spgemm_scalar_f32(int32_t a_rows, int32_t a_cols, int32_t b_cols,
+                                     const int32_t* a_row_ptr, const int32_t* a_col_idx,
+                                     const float* a_values, const int32_t* b_row_ptr,
+                                     const int32_t* b_col_idx, const float* b_values,
+                                     void* workspace, size_t workspace_bytes,
+                                     int32_t** c_row_ptr_out, int32_t** c_col_idx_out,
+                                     float** c_values_out, int32_t* c_nnz_out,
+                                     int64_t* op_counts_out) {
+  return rvsp_spgemm_hip_f32(a_rows, a_cols, b_cols, a_row_ptr, a_col_idx, a_values, b_row_ptr,
+                             b_col_idx, b_values, workspace, workspace_bytes, c_row_ptr_out,
+                             c_col_idx_out, c_values_out, c_nnz_out, op_counts_out);
 }
 
 // --- HIP extensions --------------------------------------------------------

@@ -113,6 +113,9 @@ PROTOTYPES = {
     "rvsp_spgemm_hip_f32": (C.c_int, [I32, I32, I32, P, P, P, P, P, P, P, SZ,
                                       C.POINTER(P), C.POINTER(P), C.POINTER(P),
                                       C.POINTER(I32), P]),
+    "rvsp_spgemm_scalar_f32": (C.c_int, [I32, I32, I32, P, P, P, P, P, P, P, SZ,
+                                      C.POINTER(P), C.POINTER(P), C.POINTER(P),
+                                      C.POINTER(I32), P]),
     # include/rv_sparse_hip.h
     "rvsp_spgemm_set_memory": (C.c_int, [P, C.c_int]),
     "rvsp_spgemm_set_stream": (C.c_int, [P, P]),
