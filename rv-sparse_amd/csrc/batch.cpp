@@ -110,6 +110,43 @@ struct BatchState {
   bool graph_failed = false;
   int graph_launches = 0;
   int repeats = 0;  // consecutive numeric-only computes with the same arrays
+  // Eager computes run the groups on kLanes streams at once (fork and join
+  // through events on the descriptor's stream), so one group's long-row
+  // tail overlaps the next group's work; captured computes use one stream.
+  // Two lanes: more streams than the process's hardware queues (four by
+  // default, shared with the caller's streams) made some computes stall.
+  static constexpr int kLanes = 2;
+  hipStream_t lane[kLanes - 1] = {};
+  hipEvent_t fork_ev = nullptr;
+  hipEvent_t join_ev[kLanes - 1] = {};
+
+  bool lanes_ready() {
+    if (fork_ev) return true;
+    for (int k = 0; k < kLanes - 1; ++k) {
+      if (hipStreamCreateWithFlags(&lane[k], hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&join_ev[k], hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        drop_lanes();
+        return false;
+      }
+    }
+    if (hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      drop_lanes();
+      return false;
+    }
+    return true;
+  }
+  void drop_lanes() {
+    for (int k = 0; k < kLanes - 1; ++k) {
+      if (lane[k]) (void)hipStreamSynchronize(lane[k]), (void)hipStreamDestroy(lane[k]);
+      if (join_ev[k]) (void)hipEventDestroy(join_ev[k]);
+      lane[k] = nullptr;
+      join_ev[k] = nullptr;
+    }
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    fork_ev = nullptr;
+  }
 
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -130,6 +167,7 @@ struct BatchState {
   ~BatchState() {
     drop_graph();
     drop_groups();
+    drop_lanes();
     DevBuf* bufs[] = {&a_rp, &a_ci, &a_val, &b_rp, &b_ci, &b_val, &ops, &gather_idx,
                       &gather_out};
     for (DevBuf* x : bufs) release(*x);
@@ -276,13 +314,25 @@ void build_output_tables(BatchState& bs, const rvsp_csr_matrix_t* C) {
 }
 
 // The compute body: values into the stack, every group's compute, results
-// out. Launches only, so it can be captured.
+// out. With `concurrent` the groups alternate between the descriptor's
+// stream and the batch's extra lanes (forked from and joined back into the
+// descriptor's stream); otherwise, as under capture, all run on it.
 rvsp_status_t batch_body(BatchState& bs, const rvsp_csr_matrix_t* C, bool full_out,
-                         hipStream_t s) {
+                         hipStream_t s, bool concurrent) {
   RVSP_HIP_TRY(bs.t_vals.launch(s));
   const rvsp_csr_matrix_t Bst = view((int)bs.b_row_off.back(), bs.b_cols, bs.b_nnz_off.back(),
                                      bs.b_rp.p, bs.b_ci.p, bs.b_val.p);
+  const bool conc = concurrent && bs.groups.size() > 1 && bs.lanes_ready();
+  if (conc) {
+    RVSP_HIP_TRY(hipEventRecord(bs.fork_ev, s));
+    for (int k = 0; k < BatchState::kLanes - 1; ++k)
+      RVSP_HIP_TRY(hipStreamWaitEvent(bs.lane[k], bs.fork_ev, 0));
+  }
+  int gi = 0;
   for (BatchGroup& g : bs.groups) {
+    const int lane = conc ? gi % BatchState::kLanes : 0;
+    ++gi;
+    RVSP_TRY(rvsp_spgemm_set_stream(g.inner, (void*)(lane ? bs.lane[lane - 1] : s)));
     const rvsp_csr_matrix_t Ast =
         view((int)(g.r1 - g.r0), (int)bs.b_row_off.back(),
              bs.a_nnz_off[g.p1] - bs.a_nnz_off[g.p0], (const int*)bs.a_rp.p + g.r0, bs.a_ci.p,
@@ -292,6 +342,12 @@ rvsp_status_t batch_body(BatchState& bs, const rvsp_csr_matrix_t* C, bool full_o
     rvsp_csr_matrix_t Cg = view(Ast.rows, bs.b_cols, g.c_nnz, g.c_rp.p, ci, val);
     char token = 0;
     RVSP_TRY(rvsp_spgemm_compute(g.inner, &Ast, &Bst, &Cg, &token));
+  }
+  if (conc) {
+    for (int k = 0; k < BatchState::kLanes - 1; ++k) {
+      RVSP_HIP_TRY(hipEventRecord(bs.join_ev[k], bs.lane[k]));
+      RVSP_HIP_TRY(hipStreamWaitEvent(s, bs.join_ev[k], 0));
+    }
   }
   const SegTable& out = full_out ? bs.t_out_full : bs.t_out_vals;
   if (!out.chunks.empty()) RVSP_HIP_TRY(out.launch(s));
@@ -544,7 +600,7 @@ rvsp_status_t rvsp_spgemm_batch_compute(rvsp_spgemm_descr_t descr, int32_t count
     if (!bs.exec) {
       // Capture one numeric-only compute; later ones replay it.
       RVSP_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      const rvsp_status_t st = batch_body(bs, C, false, s);
+      const rvsp_status_t st = batch_body(bs, C, false, s, false);
       hipGraph_t g = nullptr;
       const hipError_t ec = hipStreamEndCapture(s, &g);
       hipGraphExec_t ex = nullptr;
@@ -564,10 +620,10 @@ rvsp_status_t rvsp_spgemm_batch_compute(rvsp_spgemm_descr_t descr, int32_t count
       RVSP_HIP_TRY(hipGraphLaunch(bs.exec, s));
       ++bs.graph_launches;
     } else {
-      RVSP_TRY(batch_body(bs, C, false, s));
+      RVSP_TRY(batch_body(bs, C, false, s, true));
     }
   } else {
-    RVSP_TRY(batch_body(bs, C, !bs.out_filled, s));
+    RVSP_TRY(batch_body(bs, C, !bs.out_filled, s, true));
     bs.out_filled = true;
   }
   if (!d->async) RVSP_HIP_TRY(hipStreamSynchronize(s));

@@ -454,6 +454,9 @@ rvsp_status_t ensure_split(rvsp_spgemm_descr* d, const DevCsr& B, hipStream_t s)
   t.off = (const int*)d->split_off.p;
   t.sp = (const int*)d->split_sp.p;
   RVSP_HIP_TRY(launch_split_build(B, t, (int*)d->split_sp.p, s));
+  // Complete before any stream uses it (the groups of a batch may run on
+  // other streams and share this table).
+  RVSP_HIP_TRY(hipStreamSynchronize(s));
   d->split = t;
   d->split_valid = true;
   return RVSP_SUCCESS;
