@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <climits>
 #include <map>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -89,6 +90,8 @@ struct SegTable {
   }
 };
 
+constexpr int kMaxLanes = 2;
+
 struct BatchState {
   bool estimated = false;
   int count = 0;
@@ -98,7 +101,7 @@ struct BatchState {
   std::vector<long long> a_row_off, a_nnz_off, b_row_off, b_nnz_off;
   int b_cols = 1;
   DevBuf a_rp, a_ci, a_val, b_rp, b_ci, b_val;
-  DevBuf ops, gather_idx, gather_out;
+  DevBuf ops, gather_idx[kMaxLanes], gather_out[kMaxLanes];
   std::vector<BatchGroup> groups;
   std::vector<int32_t> c_nnz;
   SegTable t_struct, t_vals, t_out_full, t_out_vals;
@@ -115,7 +118,7 @@ struct BatchState {
   // tail overlaps the next group's work; captured computes use one stream.
   // Two lanes: more streams than the process's hardware queues (four by
   // default, shared with the caller's streams) made some computes stall.
-  static constexpr int kLanes = 2;
+  static constexpr int kLanes = kMaxLanes;
   hipStream_t lane[kLanes - 1] = {};
   hipEvent_t fork_ev = nullptr;
   hipEvent_t join_ev[kLanes - 1] = {};
@@ -168,9 +171,12 @@ struct BatchState {
     drop_graph();
     drop_groups();
     drop_lanes();
-    DevBuf* bufs[] = {&a_rp, &a_ci, &a_val, &b_rp, &b_ci, &b_val, &ops, &gather_idx,
-                      &gather_out};
+    DevBuf* bufs[] = {&a_rp, &a_ci, &a_val, &b_rp, &b_ci, &b_val, &ops};
     for (DevBuf* x : bufs) release(*x);
+    for (int t = 0; t < kMaxLanes; ++t) {
+      release(gather_idx[t]);
+      release(gather_out[t]);
+    }
     t_struct.release_all();
     t_vals.release_all();
     t_out_full.release_all();
@@ -489,30 +495,54 @@ rvsp_status_t rvsp_spgemm_batch_work_estimation(rvsp_spgemm_descr_t descr, int32
     if (!d->choices.empty())
       RVSP_TRY(rvsp_spgemm_set_bin_choices(g.inner, d->choices.data(),
                                            (int32_t)d->choices.size()));
-    const rvsp_csr_matrix_t Ast =
-        view((int)(g.r1 - g.r0), (int)bs.b_row_off[nb], bs.a_nnz_off[g.p1] - bs.a_nnz_off[g.p0],
-             (const int*)bs.a_rp.p + g.r0, bs.a_ci.p, bs.a_val.p);
-    size_t ws = 0;
-    RVSP_TRY(rvsp_spgemm_work_estimation(g.inner, &Ast, &Bst, &ws, &g.c_nnz));
-    // Pair boundaries inside the group's row pointers.
-    const int np = g.p1 - g.p0;
-    std::vector<long long> idx(np + 1);
-    for (int i = 0; i <= np; ++i) idx[i] = bs.a_row_off[g.p0 + i] - g.r0;
-    std::vector<int> at(np + 1);
-    RVSP_TRY(ensure(bs.gather_idx, (size_t)(np + 1) * 8));
-    RVSP_TRY(ensure(bs.gather_out, (size_t)(np + 1) * 4));
-    RVSP_HIP_TRY(hipMemcpyAsync(bs.gather_idx.p, idx.data(), (size_t)(np + 1) * 8,
-                                hipMemcpyHostToDevice, s));
-    RVSP_HIP_TRY(launch_gather_i32((const int*)g.inner->c_rp.p, (const long long*)bs.gather_idx.p,
-                                   np + 1, (int*)bs.gather_out.p, s));
-    RVSP_HIP_TRY(hipMemcpyAsync(at.data(), bs.gather_out.p, (size_t)(np + 1) * 4,
-                                hipMemcpyDeviceToHost, s));
-    RVSP_HIP_TRY(hipStreamSynchronize(s));
-    g.off.assign(np + 1, 0);
-    for (int i = 0; i <= np; ++i) g.off[i] = at[i];
-    for (int i = 0; i < np; ++i) bs.c_nnz[g.p0 + i] = at[i + 1] - at[i];
-    RVSP_TRY(ensure(g.c_rp, (size_t)(g.r1 - g.r0 + 1) * 4));
   }
+  // The groups are estimated kLanes at a time: host thread t estimates
+  // groups t, t + kLanes, ... on lane t (each estimation waits for its own
+  // counts, so the lanes' host round trips and kernels overlap). Different
+  // descriptors are independent; the library's shared tables are locked.
+  const int lanes = bs.groups.size() > 1 && bs.lanes_ready() ? BatchState::kLanes : 1;
+  auto estimate_lane = [&](int t) -> rvsp_status_t {
+    hipStream_t ls = t ? bs.lane[t - 1] : s;
+    DevBuf& gidx = bs.gather_idx[t];
+    DevBuf& gout = bs.gather_out[t];
+    for (size_t gi = t; gi < bs.groups.size(); gi += lanes) {
+      BatchGroup& g = bs.groups[gi];
+      RVSP_TRY(rvsp_spgemm_set_stream(g.inner, (void*)ls));
+      const rvsp_csr_matrix_t Ast =
+          view((int)(g.r1 - g.r0), (int)bs.b_row_off[nb], bs.a_nnz_off[g.p1] - bs.a_nnz_off[g.p0],
+               (const int*)bs.a_rp.p + g.r0, bs.a_ci.p, bs.a_val.p);
+      size_t ws = 0;
+      RVSP_TRY(rvsp_spgemm_work_estimation(g.inner, &Ast, &Bst, &ws, &g.c_nnz));
+      // Pair boundaries inside the group's row pointers.
+      const int np = g.p1 - g.p0;
+      std::vector<long long> idx(np + 1);
+      for (int i = 0; i <= np; ++i) idx[i] = bs.a_row_off[g.p0 + i] - g.r0;
+      std::vector<int> at(np + 1);
+      RVSP_TRY(ensure(gidx, (size_t)(np + 1) * 8));
+      RVSP_TRY(ensure(gout, (size_t)(np + 1) * 4));
+      RVSP_HIP_TRY(hipMemcpyAsync(gidx.p, idx.data(), (size_t)(np + 1) * 8,
+                                  hipMemcpyHostToDevice, ls));
+      RVSP_HIP_TRY(launch_gather_i32((const int*)g.inner->c_rp.p, (const long long*)gidx.p,
+                                     np + 1, (int*)gout.p, ls));
+      RVSP_HIP_TRY(hipMemcpyAsync(at.data(), gout.p, (size_t)(np + 1) * 4,
+                                  hipMemcpyDeviceToHost, ls));
+      RVSP_HIP_TRY(hipStreamSynchronize(ls));
+      g.off.assign(np + 1, 0);
+      for (int i = 0; i <= np; ++i) g.off[i] = at[i];
+      for (int i = 0; i < np; ++i) bs.c_nnz[g.p0 + i] = at[i + 1] - at[i];
+      RVSP_TRY(ensure(g.c_rp, (size_t)(g.r1 - g.r0 + 1) * 4));
+    }
+    return RVSP_SUCCESS;
+  };
+  std::vector<rvsp_status_t> lane_status(lanes, RVSP_SUCCESS);
+  {
+    std::vector<std::thread> helpers;
+    for (int t = 1; t < lanes; ++t)
+      helpers.emplace_back([&, t] { lane_status[t] = estimate_lane(t); });
+    lane_status[0] = estimate_lane(0);
+    for (std::thread& h : helpers) h.join();
+  }
+  for (rvsp_status_t st : lane_status) RVSP_TRY(st);
   for (int p = 0; p < count; ++p) c_nnz[p] = bs.c_nnz[p];
   bs.estimated = true;
   return RVSP_SUCCESS;

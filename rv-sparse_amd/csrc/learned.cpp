@@ -223,6 +223,11 @@ struct Model {
   float mu[kMlpF];
   float sigma[kMlpF];
   void* dev_params = nullptr;  // device copy for prediction
+  // Prediction scratch for one phase's bins (kept: selection runs under the
+  // lock, and an allocation per call would synchronise the device).
+  float* dev_x = nullptr;
+  unsigned long long* dev_mask = nullptr;
+  int* dev_cls = nullptr;
 };
 Model g_model;
 
@@ -447,18 +452,21 @@ rvsp_status_t select_learned(PhaseTable& t, int phase, const std::vector<BinFeat
     RVSP_HIP_TRY(hipMemcpy(g_model.dev_params, g_model.params.data(), (size_t)kMlpP * 4,
                            hipMemcpyHostToDevice));
   }
-  const int n = (int)bins.size();
-  Scratch sc;
-  float* d_x;
-  unsigned long long* d_mask;
-  int* d_cls;
-  RVSP_HIP_TRY(sc.upload(&d_x, x.data(), x.size(), s));
-  RVSP_HIP_TRY(sc.upload(&d_mask, mask.data(), mask.size(), s));
-  RVSP_HIP_TRY(sc.alloc((void**)&d_cls, (size_t)n * 4));
-  RVSP_HIP_TRY(launch_mlp_forward((const float*)g_model.dev_params, d_x, d_mask, n, nullptr,
-                                  d_cls, s));
+  const int n = (int)bins.size();  // <= kMaxBins
+  if (!g_model.dev_x) {
+    RVSP_HIP_TRY(hipMalloc(&g_model.dev_x, (size_t)kMaxBins * kMlpF * sizeof(float)));
+    RVSP_HIP_TRY(hipMalloc(&g_model.dev_mask, (size_t)kMaxBins * sizeof(unsigned long long)));
+    RVSP_HIP_TRY(hipMalloc(&g_model.dev_cls, (size_t)kMaxBins * sizeof(int)));
+  }
+  RVSP_HIP_TRY(hipMemcpyAsync(g_model.dev_x, x.data(), x.size() * sizeof(float),
+                              hipMemcpyHostToDevice, s));
+  RVSP_HIP_TRY(hipMemcpyAsync(g_model.dev_mask, mask.data(), mask.size() * sizeof(unsigned long long),
+                              hipMemcpyHostToDevice, s));
+  RVSP_HIP_TRY(launch_mlp_forward((const float*)g_model.dev_params, g_model.dev_x,
+                                  g_model.dev_mask, n, nullptr, g_model.dev_cls, s));
   std::vector<int> cls(n);
-  RVSP_HIP_TRY(hipMemcpyAsync(cls.data(), d_cls, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  RVSP_HIP_TRY(hipMemcpyAsync(cls.data(), g_model.dev_cls, (size_t)n * 4, hipMemcpyDeviceToHost,
+                              s));
   RVSP_HIP_TRY(hipStreamSynchronize(s));
   for (int i = 0; i < n; ++i) {
     if (cls[i] < 0) continue;
