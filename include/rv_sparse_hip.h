@@ -15,6 +15,7 @@
 #include <stdint.h>
 
 #include "rv_sparse.h"
+#include "rvsp_spgemm.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -199,6 +200,22 @@ rvsp_status_t rvsp_hip_free(void *ptr);
 rvsp_status_t rvsp_hip_memcpy(void *dst, const void *src, size_t bytes,
                               int kind);
 rvsp_status_t rvsp_hip_synchronize(void);
+
+/* Canonical-CSR check of device-resident arrays: the verdict and failing
+ * row that rvsp_csr_check (src/kernels/spgemm/rvsp_support.c:36-126) gives
+ * for the same arrays on the host, computed on the GPU. Rows are judged in
+ * order and the first failure wins; within a row a decreasing row pointer
+ * comes first, then per entry out of range, repeated (DUPLICATE),
+ * descending (UNSORTED). col_idx holds nnz entries: with a column array, a
+ * row pointer beyond nnz is reported as RVSP_CSR_BAD_ROW_PTR at that row
+ * (the host check has no nnz and would read past the array). *bad_row_out (may be NULL) gets the
+ * failing row, or -1. The rvsp_spgemm_csr path runs this check on its
+ * staged operands instead of scanning them on the host. Synchronous; the
+ * return value reports HIP failures only. */
+rvsp_status_t rvsp_csr_check_device(int32_t rows, int32_t cols, int32_t nnz,
+                                    const int32_t *row_ptr, const int32_t *col_idx,
+                                    rvsp_csr_status_t *status_out,
+                                    int32_t *bad_row_out);
 
 /* Number of usable gfx950 devices (0 when none; never an error for "no
  * GPU"). */

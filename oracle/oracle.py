@@ -132,6 +132,63 @@ def rlib():
     return _rlib
 
 
+# Canonical-CSR verdicts (values of rvsp_csr_status_t).
+CSR_OK, CSR_BAD_ROW_PTR, CSR_COL_OUT_OF_RANGE, CSR_UNSORTED, CSR_DUPLICATE = range(5)
+
+
+def csr_check(rows, cols, rp, ci, variant=0):
+    """(verdict, bad_row) of a CSR structure, restating the reference's
+    rvsp_csr_check (src/kernels/spgemm/rvsp_support.c:36-126): row_ptr must
+    start at 0; rows are scanned in order, a row whose end is below its start
+    is BAD_ROW_PTR at that row, a non-empty row with no column array is
+    BAD_ROW_PTR with no row named, and within a row each column must lie in
+    [0, cols) (COL_OUT_OF_RANGE), differ from the previous one (DUPLICATE) and
+    exceed it (UNSORTED); the first failure wins. rp / ci: arrays or None.
+    variant != 0 plants a mistake for the tests that pin this function:
+    1 lets a repeated column pass, 2 reports the last failing entry."""
+    if rows < 0 or cols < 0 or rp is None:
+        return CSR_BAD_ROW_PTR, -1
+    rp = np.asarray(rp, np.int64)
+    if rp[0] != 0:
+        return CSR_BAD_ROW_PTR, -1
+    s, e = rp[:rows], rp[1:rows + 1]
+    dec = np.flatnonzero(e < s)
+    limit = int(dec[0]) if dec.size else rows  # rows before it are monotone
+    tail = (CSR_BAD_ROW_PTR, limit) if limit < rows else (CSR_OK, -1)
+    lens = e[:limit] - s[:limit]
+    if ci is None:
+        return (CSR_BAD_ROW_PTR, -1) if np.any(lens > 0) else tail
+    n = int(rp[limit]) if limit > 0 else 0
+    if n == 0:
+        return tail
+    col = np.asarray(ci[:n], np.int64)
+    row_of = np.repeat(np.arange(limit), lens)
+    prev = np.empty(n, np.int64)
+    prev[0] = -1
+    prev[1:] = col[:-1]
+    prev[s[:limit][lens > 0]] = -1  # the first entry of a row has no predecessor
+    out = (col < 0) | (col >= cols)
+    bad = np.flatnonzero(out | ((col < prev) if variant == 1 else (col <= prev)))
+    if bad.size == 0:
+        return tail
+    p = int(bad[-1] if variant == 2 else bad[0])
+    kind = CSR_COL_OUT_OF_RANGE if out[p] else (CSR_DUPLICATE if col[p] == prev[p] else CSR_UNSORTED)
+    return kind, int(row_of[p])
+
+
+def ref_csr_check(rows, cols, rp, ci):
+    """The reference library's own rvsp_csr_check on host arrays."""
+    h = rlib()
+    h.rvsp_csr_check.argtypes = [_I32, _I32, _P, _P, C.POINTER(_I32)]
+    h.rvsp_csr_check.restype = C.c_int
+    rpa = None if rp is None else np.ascontiguousarray(rp, np.int32)
+    cia = None if ci is None else np.ascontiguousarray(ci if len(ci) else [0], np.int32)
+    bad = _I32(-7)
+    st = h.rvsp_csr_check(rows, cols, None if rpa is None else rpa.ctypes.data,
+                          None if cia is None else cia.ctypes.data, C.byref(bad))
+    return st, bad.value
+
+
 _libc = C.CDLL(None)
 _libc.free.argtypes = [_P]
 

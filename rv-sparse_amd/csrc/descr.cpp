@@ -316,6 +316,70 @@ rvsp_status_t plan_keep(rvsp_spgemm_descr* d, int M, hipStream_t s) {
   return RVSP_SUCCESS;
 }
 
+// rvsp_csr_check's verdict for device arrays, found on the GPU: a row pass
+// (row pointers; a pointer past nnz counts as malformed, since no more
+// entries exist), then an entry pass over the rows before the first
+// malformed pointer, each proposing its failure as (row << 32) | code; the
+// smallest is where the reference's in-order scan stops. The kind of an
+// entry failure is then read off that entry and its predecessor.
+rvsp_status_t csr_check_device(int rows, int cols, int nnz, const int* rp, const int* ci,
+                               DevBuf& key_buf, hipStream_t s, rvsp_csr_status_t* verdict,
+                               int* bad_row) {
+  *verdict = RVSP_CSR_OK;
+  *bad_row = -1;
+  if (rows < 0 || cols < 0 || nnz < 0 || rp == NULL) {
+    *verdict = RVSP_CSR_BAD_ROW_PTR;
+    return RVSP_SUCCESS;
+  }
+  int head = 0;
+  RVSP_HIP_TRY(hipMemcpyAsync(&head, rp, 4, hipMemcpyDeviceToHost, s));
+  RVSP_HIP_TRY(hipStreamSynchronize(s));
+  if (head != 0) {
+    *verdict = RVSP_CSR_BAD_ROW_PTR;
+    return RVSP_SUCCESS;
+  }
+  if (rows == 0) return RVSP_SUCCESS;
+  RVSP_TRY(ensure(key_buf, 8));
+  unsigned long long* dkey = (unsigned long long*)key_buf.p;
+  unsigned long long key = ~0ull;
+  RVSP_HIP_TRY(hipMemcpyAsync(dkey, &key, 8, hipMemcpyHostToDevice, s));
+  RVSP_HIP_TRY(launch_csr_check_rows(rows, nnz, rp, ci != NULL, dkey, s));
+  RVSP_HIP_TRY(hipMemcpyAsync(&key, dkey, 8, hipMemcpyDeviceToHost, s));
+  RVSP_HIP_TRY(hipStreamSynchronize(s));
+  const int limit = key == ~0ull ? rows : (int)(key >> 32);
+  if (ci != NULL && limit > 0) {
+    RVSP_HIP_TRY(launch_csr_check_cols(limit, cols, nnz, rp, ci, dkey, s));
+    RVSP_HIP_TRY(hipMemcpyAsync(&key, dkey, 8, hipMemcpyDeviceToHost, s));
+    RVSP_HIP_TRY(hipStreamSynchronize(s));
+  }
+  if (key == ~0ull) return RVSP_SUCCESS;
+  const int row = (int)(key >> 32);
+  const unsigned code = (unsigned)(key & 0xffffffffu);
+  *verdict = RVSP_CSR_BAD_ROW_PTR;
+  if (code == 0) *bad_row = row;
+  if (code < 2) return RVSP_SUCCESS;  // code 1: no column array (no row named)
+  // Entry j of the row failed with every earlier entry passing.
+  const int j = (int)(code - 2);
+  int start = 0, pair[2] = {-1, -1};
+  RVSP_HIP_TRY(hipMemcpyAsync(&start, rp + row, 4, hipMemcpyDeviceToHost, s));
+  RVSP_HIP_TRY(hipStreamSynchronize(s));
+  const int p = start + j;
+  if (j > 0)
+    RVSP_HIP_TRY(hipMemcpyAsync(pair, ci + p - 1, 8, hipMemcpyDeviceToHost, s));
+  else
+    RVSP_HIP_TRY(hipMemcpyAsync(pair + 1, ci + p, 4, hipMemcpyDeviceToHost, s));
+  RVSP_HIP_TRY(hipStreamSynchronize(s));
+  const int prev = pair[0], c = pair[1];
+  *bad_row = row;
+  if (c < 0 || c >= cols)
+    *verdict = RVSP_CSR_COL_OUT_OF_RANGE;
+  else if (c == prev)
+    *verdict = RVSP_CSR_DUPLICATE;
+  else
+    *verdict = RVSP_CSR_UNSORTED;
+  return RVSP_SUCCESS;
+}
+
 // The symbolic phase on the device: analysis, binning, exact count, scan,
 // then the fill and numeric plans. On success *total holds nnz(C).
 rvsp_status_t estimate(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
@@ -609,7 +673,8 @@ void rvsp_spgemm_descr_destroy(rvsp_spgemm_descr_t descr) {
                     &d->fill.rows, &d->numeric.rows, &d->c_ci_dev, &d->c_val_dev,
                     &d->gbits, &d->gdense, &d->learn_feat, &d->bin_stats,
                     &d->map, &d->poff, &d->map_weights, &d->keep, &d->koff,
-                    &d->keep_w, &d->split_cnt, &d->split_off, &d->split_sp};
+                    &d->keep_w, &d->split_cnt, &d->split_off, &d->split_sp,
+                    &d->check_key};
   for (DevBuf* b : bufs) release(*b);
   destroy_graph(d);
   destroy_batch(d);
@@ -659,6 +724,16 @@ rvsp_status_t rvsp_spgemm_work_estimation(rvsp_spgemm_descr_t descr,
   RVSP_TRY(descr_stream(d, &s));
   DevCsr dA, dB;
   RVSP_TRY(operand_views(d, A, B, true, false, &dA, &dB, s));
+  if (d->check_csr) {
+    rvsp_csr_status_t va = RVSP_CSR_OK, vb = RVSP_CSR_OK;
+    int bad = -1;
+    RVSP_TRY(csr_check_device(A->rows, A->cols, A->nnz, dA.rp, dA.ci, d->check_key, s, &va,
+                              &bad));
+    if (va == RVSP_CSR_OK && dB.rp != dA.rp)
+      RVSP_TRY(csr_check_device(B->rows, B->cols, B->nnz, dB.rp, dB.ci, d->check_key, s, &vb,
+                                &bad));
+    if (va != RVSP_CSR_OK || vb != RVSP_CSR_OK) return RVSP_ERROR_INVALID_CSR;
+  }
   long long total = 0;
   rvsp_status_t st = estimate(d, dA, dB, &total, s);
   if (st != RVSP_SUCCESS) {
@@ -777,12 +852,21 @@ static rvsp_status_t oneshot(const rvsp_csr_matrix_t* A, const rvsp_csr_matrix_t
   if (A->dtype != RVSP_DTYPE_FP32 || B->dtype != RVSP_DTYPE_FP32)
     return RVSP_ERROR_UNSUPPORTED_DTYPE;
   if (A->cols != B->rows) return RVSP_ERROR_INVALID_ARGUMENT;
-  if (rvsp_csr_check(A->rows, A->cols, A->row_ptr, A->col_idx, NULL) != RVSP_CSR_OK ||
-      rvsp_csr_check(B->rows, B->cols, B->row_ptr, B->col_idx, NULL) != RVSP_CSR_OK)
+  // The canonical check runs on the device once the estimation has staged
+  // the operands, before any kernel reads them. Operands the estimation
+  // would refuse first, or a process without a usable device, are checked
+  // here on the host, so the reference's order (INVALID_CSR before the
+  // descriptor's own argument errors and before device errors) holds.
+  const bool on_device = check_operand(A) == RVSP_SUCCESS &&
+                         check_operand(B) == RVSP_SUCCESS && device_ready() == RVSP_SUCCESS;
+  if (!on_device &&
+      (rvsp_csr_check(A->rows, A->cols, A->row_ptr, A->col_idx, NULL) != RVSP_CSR_OK ||
+       rvsp_csr_check(B->rows, B->cols, B->row_ptr, B->col_idx, NULL) != RVSP_CSR_OK))
     return RVSP_ERROR_INVALID_CSR;
   rvsp_spgemm_descr_t d = NULL;
   rvsp_status_t st = rvsp_spgemm_descr_create(&d);
   if (st != RVSP_SUCCESS) return st;
+  d->check_csr = on_device;
   st = rvsp_spgemm_set_algo(d, algo);
   if (st != RVSP_SUCCESS) {
     rvsp_spgemm_descr_destroy(d);
@@ -1066,6 +1150,20 @@ rvsp_status_t rvsp_hip_synchronize(void) {
   RVSP_TRY(device_ready());
   RVSP_HIP_TRY(hipDeviceSynchronize());
   return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_csr_check_device(int32_t rows, int32_t cols, int32_t nnz,
+                                    const int32_t* row_ptr, const int32_t* col_idx,
+                                    rvsp_csr_status_t* status_out, int32_t* bad_row_out) {
+  if (status_out == NULL) return RVSP_ERROR_NULL_POINTER;
+  RVSP_TRY(device_ready());
+  DevBuf key;
+  int bad = -1;
+  const rvsp_status_t st =
+      csr_check_device(rows, cols, nnz, row_ptr, col_idx, key, nullptr, status_out, &bad);
+  release(key);
+  if (bad_row_out) *bad_row_out = bad;
+  return st;
 }
 
 int32_t rvsp_hip_device_count(void) {

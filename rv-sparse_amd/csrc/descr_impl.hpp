@@ -101,6 +101,11 @@ struct rvsp_spgemm_descr {
   bool batch_graph = true;
   long long batch_limit = 0;  // products per group; 0 = INT32_MAX
 
+  // Set by the one-shot path: work_estimation checks the staged operands
+  // for canonical CSR on the device before any other kernel reads them.
+  bool check_csr = false;
+  rvsp::DevBuf check_key;
+
   // Sorted columns of the count phase's LDS-bitmap rows, kept for the fill
   // phase (offsets koff: exclusive scan of min(Op_i, span_i) over them).
   rvsp::DevBuf keep, koff, keep_w;

@@ -647,6 +647,86 @@ hipError_t launch_count_bin(const BinSpec& b, const int* rows, int nrows,
 }
 
 // ---------------------------------------------------------------------------
+// Canonical-CSR check on the device (reference: rvsp_support.c:36-126). Each
+// failing row pointer or entry proposes (row << 32) | code; the smallest
+// proposal is the failure a scan of rows in order, entries in order, would
+// stop at. An entry is judged against its predecessor in the row only: at
+// the first failure every earlier entry passed, which is all the scan needs.
+
+__global__ __launch_bounds__(kBlock) void k_csr_check_rows(
+    int rows, int nnz, const int* __restrict__ rp, int has_cols,
+    unsigned long long* __restrict__ key) {
+  for (int r = blockIdx.x * kBlock + threadIdx.x; r < rows; r += gridDim.x * kBlock) {
+    const int s = rp[r], e = rp[r + 1];
+    int code = -1;
+    if (e < s)
+      code = 0;
+    else if (e > s && !has_cols)
+      code = 1;
+    else if (has_cols && e > nnz)
+      code = 0;  // past the column array (without one nothing is read)
+    if (code >= 0) atomicMin(key, ((unsigned long long)r << 32) | (unsigned)code);
+  }
+}
+
+constexpr int kCheckTile = 4 * kBlock;  // entries per workgroup step
+
+// Last row r in [lo, hi] with rp[r] <= p (rp[lo] <= p holds).
+__device__ __forceinline__ int row_of_entry(const int* __restrict__ rp, int lo, int hi, int p) {
+  while (lo < hi) {
+    const int mid = lo + ((hi - lo + 1) >> 1);
+    if (rp[mid] <= p)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_csr_check_cols(
+    int limit, int cols, const int* __restrict__ rp, const int* __restrict__ ci,
+    unsigned long long* __restrict__ key) {
+  __shared__ int range[2];
+  const int P = rp[limit];  // entries of rows [0, limit)
+  for (long long b = (long long)blockIdx.x * kCheckTile; b < P;
+       b += (long long)gridDim.x * kCheckTile) {
+    const int b0 = (int)b;
+    const int b1 = (int)min((long long)P, b + kCheckTile) - 1;
+    // Rows of the step's first and last entries bound every search below.
+    if (threadIdx.x < 2) range[threadIdx.x] = row_of_entry(rp, 0, limit - 1, threadIdx.x ? b1 : b0);
+    __syncthreads();
+    const int rlo = range[0], rhi = range[1];
+    for (int p = b0 + threadIdx.x; p <= b1; p += kBlock) {
+      const int r = row_of_entry(rp, rlo, rhi, p);
+      const int s = rp[r];
+      const int c = ci[p];
+      bool bad = c < 0 || c >= cols;
+      if (!bad && p > s) bad = c <= ci[p - 1];
+      if (bad) atomicMin(key, ((unsigned long long)r << 32) | (2u + (unsigned)(p - s)));
+    }
+    __syncthreads();  // range is rewritten by the next step
+  }
+}
+
+hipError_t launch_csr_check_rows(int rows, int nnz, const int* row_ptr, bool has_cols,
+                                 unsigned long long* key, hipStream_t s) {
+  if (rows <= 0) return hipSuccess;
+  const int grid = std::min((rows + kBlock - 1) / kBlock, 8192);
+  hipLaunchKernelGGL(k_csr_check_rows, dim3(grid), dim3(kBlock), 0, s, rows, nnz, row_ptr,
+                     has_cols ? 1 : 0, key);
+  return hipGetLastError();
+}
+
+hipError_t launch_csr_check_cols(int limit, int cols, int nnz, const int* row_ptr,
+                                 const int* col_idx, unsigned long long* key, hipStream_t s) {
+  if (limit <= 0 || nnz <= 0) return hipSuccess;
+  const int grid = std::min((nnz + kCheckTile - 1) / kCheckTile, 8192);
+  hipLaunchKernelGGL(k_csr_check_cols, dim3(grid), dim3(kBlock), 0, s, limit, cols, row_ptr,
+                     col_idx, key);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Exclusive scan of the per-row counts into C.row_ptr (reference:
 // rvsp_symbolic.c:75-85). Sums are kept in 64 bits so the caller can apply
 // the reference's INT32_MAX guard (rvsp_symbolic.c:69-72) to the total.

@@ -131,6 +131,18 @@ hipError_t launch_copy_kept(const int* rows, int nrows, const int* keep,
                             hipStream_t s);
 hipError_t launch_scan(const int* cnt, int rows, int* c_rp, long long* total,
                        long long* tile_sums, hipStream_t s);
+// Canonical-CSR check of device arrays (reference rvsp_support.c:36-126).
+// Failures are keyed (row << 32) | code and *key keeps the smallest (the
+// caller sets it to ~0 first): code 0 a row pointer that decreases or (with
+// column indices) passes nnz, 1 a non-empty row without column indices,
+// 2 + j the row's entry j
+// (out of range, or not above entry j - 1). The row pass covers all rows;
+// the column pass the entries of rows [0, limit), which it reads from
+// row_ptr[limit] (<= nnz when the row pass found nothing before limit).
+hipError_t launch_csr_check_rows(int rows, int nnz, const int* row_ptr, bool has_cols,
+                                 unsigned long long* key, hipStream_t s);
+hipError_t launch_csr_check_cols(int limit, int cols, int nnz, const int* row_ptr,
+                                 const int* col_idx, unsigned long long* key, hipStream_t s);
 size_t scan_tile_count(int rows);
 hipError_t launch_fill_bin(const BinSpec& b, const int* rows, int nrows,
                            const DevCsr& A, const DevCsr& B, RowStats st,

@@ -92,6 +92,15 @@ def synchronize():
     check(lib().rvsp_hip_synchronize(), "synchronize")
 
 
+def csr_check_device(rows, cols, nnz, row_ptr, col_idx):
+    """rvsp_csr_check_device on device pointers (ints; col_idx may be None):
+    returns (verdict, bad_row) with rvsp_csr_check's meaning."""
+    st, bad = C.c_int32(-1), C.c_int32(-7)
+    check(lib().rvsp_csr_check_device(rows, cols, nnz, row_ptr, col_idx, C.byref(st),
+                                      C.byref(bad)), "csr_check_device")
+    return st.value, bad.value
+
+
 def device_count():
     return int(lib().rvsp_hip_device_count())
 

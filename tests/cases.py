@@ -105,3 +105,76 @@ def laplacian_squared_nnz(g):
     offsets = [(0, 0), (1, 0), (-1, 0), (0, 1), (0, -1), (2, 0), (-2, 0), (0, 2), (0, -2),
                (1, 1), (1, -1), (-1, 1), (-1, -1)]
     return sum((g - abs(dx)) * (g - abs(dy)) for dx, dy in offsets)
+
+
+# ---------------------------------------------------------------------------
+# Canonical-CSR check cases: seeded valid structures (empty rows, rows longer
+# than one workgroup step of the device check) and the same structures with
+# planted faults. Every row pointer stays within [0, nnz], so the reference's
+# host check never reads past the column array.
+
+def _csr_base(rng, rows, cols):
+    lens = rng.poisson(4.0, size=rows)
+    lens[rng.random(rows) < 0.15] = 0
+    long_rows = rng.choice(rows, size=max(1, rows // 60), replace=False)
+    lens[long_rows] = rng.integers(900, 2600, size=long_rows.size)
+    lens = np.minimum(lens, cols)
+    rp, ci = gen.random_rows(rows, cols, lens, rng)
+    return np.asarray(rp, np.int32), np.asarray(ci, np.int32)
+
+
+def _plant(rng, rp, ci, cols, kind):
+    rp, ci = rp.copy(), ci.copy()
+    rows = rp.size - 1
+    nonempty = np.flatnonzero(np.diff(rp) > 0)
+    multi = np.flatnonzero(np.diff(rp) > 1)
+    if kind == "swap" and multi.size:
+        r = rng.choice(multi)
+        p = rng.integers(rp[r], rp[r + 1] - 1)
+        ci[p], ci[p + 1] = ci[p + 1], ci[p]
+    elif kind == "dup" and multi.size:
+        r = rng.choice(multi)
+        p = rng.integers(rp[r] + 1, rp[r + 1])
+        ci[p] = ci[p - 1]
+    elif kind == "high" and nonempty.size:
+        ci[rng.integers(0, ci.size)] = cols + int(rng.integers(0, 3))
+    elif kind == "low" and nonempty.size:
+        ci[rng.integers(0, ci.size)] = -1 - int(rng.integers(0, 3))
+    elif kind == "descend":
+        r = int(rng.integers(0, rows))
+        if rp[r] > 0:
+            rp[r + 1] = rp[r] - 1
+    elif kind == "head":
+        rp[0] = 1
+    return rp, ci
+
+
+def csr_check_cases(n_seeds=12):
+    """(name, rows, cols, row_ptr, col_idx) tuples; col_idx may be None."""
+    out = []
+    kinds = ["swap", "dup", "high", "low", "descend"]
+    for seed in range(n_seeds):
+        rng = np.random.Generator(np.random.PCG64(1000 + seed))
+        rows = int(rng.integers(50, 400))
+        cols = int(rng.integers(2700, 6000))
+        rp, ci = _csr_base(rng, rows, cols)
+        out.append((f"valid_{seed}", rows, cols, rp, ci))
+        for k in kinds:
+            out.append((f"{k}_{seed}", rows, cols) + _plant(rng, rp, ci, cols, k))
+        # Two or three faults at once: the first in scan order must win.
+        r2, c2 = rp, ci
+        for k in rng.choice(kinds, size=int(rng.integers(2, 4))):
+            r2, c2 = _plant(rng, r2, c2, cols, str(k))
+        out.append((f"multi_{seed}", rows, cols, r2, c2))
+    rng = np.random.Generator(np.random.PCG64(99))
+    rp, ci = _csr_base(rng, 120, 3000)
+    out.append(("head", 120, 3000) + _plant(rng, rp, ci, 3000, "head"))
+    out.append(("no_rows", 0, 5, np.zeros(1, np.int32), np.zeros(0, np.int32)))
+    out.append(("all_empty", 7, 5, np.zeros(8, np.int32), np.zeros(0, np.int32)))
+    out.append(("no_cols_empty", 4, 5, np.zeros(5, np.int32), None))
+    out.append(("no_cols_nonempty", 4, 5, np.array([0, 0, 2, 2, 3], np.int32), None))
+    out.append(("no_cols_descend_first", 4, 5, np.array([0, 0, 0, -1, 0], np.int32), None))
+    out.append(("one_entry", 1, 1, np.array([0, 1], np.int32), np.array([0], np.int32)))
+    out.append(("first_col_negative", 2, 4, np.array([0, 1, 2], np.int32),
+                np.array([-1, 0], np.int32)))
+    return out
