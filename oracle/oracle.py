@@ -50,8 +50,9 @@ def olib():
         h.oracle_numeric_f32.argtypes = [_I32, _I32, _I32] + [_P] * 9
         h.oracle_numeric_f32_variant.argtypes = [_I32, _I32, _I32] + [_P] * 9 + [C.c_int]
         h.oracle_numeric_f64.argtypes = [_I32, _I32, _I32] + [_P] * 10
+        h.oracle_numeric_legacy.argtypes = [_I32, _I32, _I32, C.c_int] + [_P] * 9 + [C.c_int]
         for f in ("oracle_symbolic_count", "oracle_symbolic_fill", "oracle_numeric_f32",
-                  "oracle_numeric_f32_variant", "oracle_numeric_f64"):
+                  "oracle_numeric_f32_variant", "oracle_numeric_f64", "oracle_numeric_legacy"):
             getattr(h, f).restype = C.c_int
         _olib = h
     return _olib
@@ -92,6 +93,57 @@ def spgemm(A, B, r0=0, r1=None, variant=0):
                                     _ptr(B.values), _ptr(rp), _ptr(ci), _ptr(val), variant):
         raise MemoryError("oracle allocation failed")
     return rp, ci[:nnz], val[:nnz]
+
+
+def spgemm_legacy(A, B, variant=0):
+    """(row_ptr, col_idx, values) of the legacy-dtype product, reference
+    semantics (src/kernels/spgemm/legacy/csr_scalar_f64.c and
+    csr_scalar_i8.c): A, B with float64 values give float64 C, int8 values
+    give int32 C; columns ascending, exact-zero sums dropped."""
+    i8 = A.values.dtype == np.int8
+    rp, _ = symbolic(A, B)
+    nnz = int(rp[-1])
+    ci = np.zeros(max(nnz, 1), dtype=np.int32)
+    val = np.zeros(max(nnz, 1), dtype=np.int32 if i8 else np.float64)
+    h = olib()
+    if h.oracle_symbolic_fill(0, A.rows, B.cols, _ptr(A.row_ptr), _ptr(A.col_idx),
+                              _ptr(B.row_ptr), _ptr(B.col_idx), _ptr(rp), _ptr(ci)):
+        raise MemoryError("oracle allocation failed")
+    if h.oracle_numeric_legacy(0, A.rows, B.cols, int(i8), _ptr(A.row_ptr), _ptr(A.col_idx),
+                               _ptr(A.values), _ptr(B.row_ptr), _ptr(B.col_idx),
+                               _ptr(B.values), _ptr(rp), _ptr(ci), _ptr(val), variant):
+        raise MemoryError("oracle allocation failed")
+    ci, val = ci[:nnz], val[:nnz]
+    keep = val != 0
+    row_of = np.repeat(np.arange(A.rows), np.diff(rp))
+    out_rp = np.zeros(A.rows + 1, dtype=np.int64)
+    out_rp[1:] = np.cumsum(np.bincount(row_of[keep], minlength=A.rows))
+    return out_rp.astype(np.int32), ci[keep], val[keep]
+
+
+def ref_spgemm_legacy(A, B):
+    """The reference library's own rvsp_spgemm_csr_scalar_f64 / _i8."""
+    h = rlib()
+    i8 = A.values.dtype == np.int8
+    fn = h.rvsp_spgemm_csr_scalar_i8 if i8 else h.rvsp_spgemm_csr_scalar_f64
+    fn.argtypes = [_P, _P, _P]
+    fn.restype = C.c_int
+    from rvsp import _ffi as F
+    a, b, c = A.struct(), B.struct(), F.CsrMatrix()
+    st = fn(C.byref(a), C.byref(b), C.byref(c))
+    if st:
+        return st, None
+    try:
+        n = c.nnz
+        vt = C.c_int32 if i8 else C.c_double
+        rp = np.ctypeslib.as_array((C.c_int32 * (c.rows + 1)).from_address(c.row_ptr)).copy()
+        ci = np.ctypeslib.as_array((C.c_int32 * max(n, 1)).from_address(c.col_idx))[:n].copy()
+        val = np.ctypeslib.as_array((vt * max(n, 1)).from_address(c.values))[:n].copy()
+    finally:
+        _libc.free(C.c_void_p(c.row_ptr))
+        _libc.free(C.c_void_p(c.col_idx))
+        _libc.free(C.c_void_p(c.values))
+    return 0, (rp, ci, val)
 
 
 def spgemm_f64(A, B, rp, ci, r0=0):

@@ -226,3 +226,55 @@ int oracle_numeric_f64(int32_t r0, int32_t r1, int32_t b_cols,
     free(mag);
     return 0;
 }
+
+/*
+ * Values of the legacy dtypes on the structure (c_rp, c_ci) of rows
+ * [r0, r1), zeros included (the caller drops them): the reference's legacy
+ * kernels, src/kernels/spgemm/legacy/csr_scalar_f64.c:214-236 (FP64
+ * product, then FP64 sum) and csr_scalar_i8.c:140-161 (int32 product and
+ * sum; kept here in unsigned arithmetic so a wrap is defined), both over A's
+ * entries in order and each B row's entries in order. variant 1 takes A's
+ * entries in descending order (a planted mistake for the tests).
+ */
+int oracle_numeric_legacy(int32_t r0, int32_t r1, int32_t b_cols, int i8,
+                          const int32_t *a_rp, const int32_t *a_ci, const void *a_val,
+                          const int32_t *b_rp, const int32_t *b_ci, const void *b_val,
+                          const int32_t *c_rp, const int32_t *c_ci, void *c_val,
+                          int variant)
+{
+    const size_t n = (size_t)(b_cols > 0 ? b_cols : 1);
+    double *accd = i8 ? NULL : (double *)calloc(n, sizeof(double));
+    uint32_t *acci = i8 ? (uint32_t *)calloc(n, sizeof(uint32_t)) : NULL;
+    if (!accd && !acci) return ORACLE_NOMEM;
+    for (int32_t row = r0; row < r1; ++row) {
+        const int32_t s = a_rp[row], e = a_rp[row + 1];
+        for (int32_t t = 0; t < e - s; ++t) {
+            const int32_t ap = variant == 1 ? e - 1 - t : s + t;
+            const int32_t k = a_ci[ap];
+            for (int32_t bp = b_rp[k]; bp < b_rp[k + 1]; ++bp) {
+                const int32_t col = b_ci[bp];
+                if (i8) {
+                    const int32_t prod = (int32_t)((const int8_t *)a_val)[ap] *
+                                         (int32_t)((const int8_t *)b_val)[bp];
+                    acci[col] += (uint32_t)prod;
+                } else {
+                    const double prod = ((const double *)a_val)[ap] * ((const double *)b_val)[bp];
+                    accd[col] = accd[col] + prod;
+                }
+            }
+        }
+        for (int32_t p = c_rp[row - r0]; p < c_rp[row - r0 + 1]; ++p) {
+            const int32_t col = c_ci[p];
+            if (i8) {
+                ((int32_t *)c_val)[p] = (int32_t)acci[col];
+                acci[col] = 0;
+            } else {
+                ((double *)c_val)[p] = accd[col];
+                accd[col] = 0.0;
+            }
+        }
+    }
+    free(accd);
+    free(acci);
+    return 0;
+}

@@ -97,6 +97,70 @@ RowStats row_stats(rvsp_spgemm_descr* d) {
   return st;
 }
 
+// rvsp_csr_check's verdict for device arrays, found on the GPU: a row pass
+// (row pointers; a pointer past nnz counts as malformed, since no more
+// entries exist), then an entry pass over the rows before the first
+// malformed pointer, each proposing its failure as (row << 32) | code; the
+// smallest is where the reference's in-order scan stops. The kind of an
+// entry failure is then read off that entry and its predecessor.
+rvsp_status_t csr_check_device(int rows, int cols, int nnz, const int* rp, const int* ci,
+                               DevBuf& key_buf, hipStream_t s, rvsp_csr_status_t* verdict,
+                               int* bad_row) {
+  *verdict = RVSP_CSR_OK;
+  *bad_row = -1;
+  if (rows < 0 || cols < 0 || nnz < 0 || rp == NULL) {
+    *verdict = RVSP_CSR_BAD_ROW_PTR;
+    return RVSP_SUCCESS;
+  }
+  int head = 0;
+  RVSP_HIP_TRY(hipMemcpyAsync(&head, rp, 4, hipMemcpyDeviceToHost, s));
+  RVSP_HIP_TRY(hipStreamSynchronize(s));
+  if (head != 0) {
+    *verdict = RVSP_CSR_BAD_ROW_PTR;
+    return RVSP_SUCCESS;
+  }
+  if (rows == 0) return RVSP_SUCCESS;
+  RVSP_TRY(ensure(key_buf, 8));
+  unsigned long long* dkey = (unsigned long long*)key_buf.p;
+  unsigned long long key = ~0ull;
+  RVSP_HIP_TRY(hipMemcpyAsync(dkey, &key, 8, hipMemcpyHostToDevice, s));
+  RVSP_HIP_TRY(launch_csr_check_rows(rows, nnz, rp, ci != NULL, dkey, s));
+  RVSP_HIP_TRY(hipMemcpyAsync(&key, dkey, 8, hipMemcpyDeviceToHost, s));
+  RVSP_HIP_TRY(hipStreamSynchronize(s));
+  const int limit = key == ~0ull ? rows : (int)(key >> 32);
+  if (ci != NULL && limit > 0) {
+    RVSP_HIP_TRY(launch_csr_check_cols(limit, cols, nnz, rp, ci, dkey, s));
+    RVSP_HIP_TRY(hipMemcpyAsync(&key, dkey, 8, hipMemcpyDeviceToHost, s));
+    RVSP_HIP_TRY(hipStreamSynchronize(s));
+  }
+  if (key == ~0ull) return RVSP_SUCCESS;
+  const int row = (int)(key >> 32);
+  const unsigned code = (unsigned)(key & 0xffffffffu);
+  *verdict = RVSP_CSR_BAD_ROW_PTR;
+  if (code == 0) *bad_row = row;
+  if (code < 2) return RVSP_SUCCESS;  // code 1: no column array (no row named)
+  // Entry j of the row failed with every earlier entry passing.
+  const int j = (int)(code - 2);
+  int start = 0, pair[2] = {-1, -1};
+  RVSP_HIP_TRY(hipMemcpyAsync(&start, rp + row, 4, hipMemcpyDeviceToHost, s));
+  RVSP_HIP_TRY(hipStreamSynchronize(s));
+  const int p = start + j;
+  if (j > 0)
+    RVSP_HIP_TRY(hipMemcpyAsync(pair, ci + p - 1, 8, hipMemcpyDeviceToHost, s));
+  else
+    RVSP_HIP_TRY(hipMemcpyAsync(pair + 1, ci + p, 4, hipMemcpyDeviceToHost, s));
+  RVSP_HIP_TRY(hipStreamSynchronize(s));
+  const int prev = pair[0], c = pair[1];
+  *bad_row = row;
+  if (c < 0 || c >= cols)
+    *verdict = RVSP_CSR_COL_OUT_OF_RANGE;
+  else if (c == prev)
+    *verdict = RVSP_CSR_DUPLICATE;
+  else
+    *verdict = RVSP_CSR_UNSORTED;
+  return RVSP_SUCCESS;
+}
+
 }  // namespace rvsp
 
 namespace {
@@ -313,70 +377,6 @@ rvsp_status_t plan_keep(rvsp_spgemm_descr* d, int M, hipStream_t s) {
   if (bytes > budget || ensure(d->keep, bytes) != RVSP_SUCCESS) return RVSP_SUCCESS;
   d->keep_bins = mask;
   d->keep_ok = true;
-  return RVSP_SUCCESS;
-}
-
-// rvsp_csr_check's verdict for device arrays, found on the GPU: a row pass
-// (row pointers; a pointer past nnz counts as malformed, since no more
-// entries exist), then an entry pass over the rows before the first
-// malformed pointer, each proposing its failure as (row << 32) | code; the
-// smallest is where the reference's in-order scan stops. The kind of an
-// entry failure is then read off that entry and its predecessor.
-rvsp_status_t csr_check_device(int rows, int cols, int nnz, const int* rp, const int* ci,
-                               DevBuf& key_buf, hipStream_t s, rvsp_csr_status_t* verdict,
-                               int* bad_row) {
-  *verdict = RVSP_CSR_OK;
-  *bad_row = -1;
-  if (rows < 0 || cols < 0 || nnz < 0 || rp == NULL) {
-    *verdict = RVSP_CSR_BAD_ROW_PTR;
-    return RVSP_SUCCESS;
-  }
-  int head = 0;
-  RVSP_HIP_TRY(hipMemcpyAsync(&head, rp, 4, hipMemcpyDeviceToHost, s));
-  RVSP_HIP_TRY(hipStreamSynchronize(s));
-  if (head != 0) {
-    *verdict = RVSP_CSR_BAD_ROW_PTR;
-    return RVSP_SUCCESS;
-  }
-  if (rows == 0) return RVSP_SUCCESS;
-  RVSP_TRY(ensure(key_buf, 8));
-  unsigned long long* dkey = (unsigned long long*)key_buf.p;
-  unsigned long long key = ~0ull;
-  RVSP_HIP_TRY(hipMemcpyAsync(dkey, &key, 8, hipMemcpyHostToDevice, s));
-  RVSP_HIP_TRY(launch_csr_check_rows(rows, nnz, rp, ci != NULL, dkey, s));
-  RVSP_HIP_TRY(hipMemcpyAsync(&key, dkey, 8, hipMemcpyDeviceToHost, s));
-  RVSP_HIP_TRY(hipStreamSynchronize(s));
-  const int limit = key == ~0ull ? rows : (int)(key >> 32);
-  if (ci != NULL && limit > 0) {
-    RVSP_HIP_TRY(launch_csr_check_cols(limit, cols, nnz, rp, ci, dkey, s));
-    RVSP_HIP_TRY(hipMemcpyAsync(&key, dkey, 8, hipMemcpyDeviceToHost, s));
-    RVSP_HIP_TRY(hipStreamSynchronize(s));
-  }
-  if (key == ~0ull) return RVSP_SUCCESS;
-  const int row = (int)(key >> 32);
-  const unsigned code = (unsigned)(key & 0xffffffffu);
-  *verdict = RVSP_CSR_BAD_ROW_PTR;
-  if (code == 0) *bad_row = row;
-  if (code < 2) return RVSP_SUCCESS;  // code 1: no column array (no row named)
-  // Entry j of the row failed with every earlier entry passing.
-  const int j = (int)(code - 2);
-  int start = 0, pair[2] = {-1, -1};
-  RVSP_HIP_TRY(hipMemcpyAsync(&start, rp + row, 4, hipMemcpyDeviceToHost, s));
-  RVSP_HIP_TRY(hipStreamSynchronize(s));
-  const int p = start + j;
-  if (j > 0)
-    RVSP_HIP_TRY(hipMemcpyAsync(pair, ci + p - 1, 8, hipMemcpyDeviceToHost, s));
-  else
-    RVSP_HIP_TRY(hipMemcpyAsync(pair + 1, ci + p, 4, hipMemcpyDeviceToHost, s));
-  RVSP_HIP_TRY(hipStreamSynchronize(s));
-  const int prev = pair[0], c = pair[1];
-  *bad_row = row;
-  if (c < 0 || c >= cols)
-    *verdict = RVSP_CSR_COL_OUT_OF_RANGE;
-  else if (c == prev)
-    *verdict = RVSP_CSR_DUPLICATE;
-  else
-    *verdict = RVSP_CSR_UNSORTED;
   return RVSP_SUCCESS;
 }
 
@@ -934,6 +934,14 @@ rvsp_status_t rvsp_spgemm_csr(const rvsp_csr_matrix_t* A, const rvsp_csr_matrix_
   if (backend == RVSP_BACKEND_RVV_INTRINSICS && A->dtype == RVSP_DTYPE_FP32 &&
       B->dtype == RVSP_DTYPE_FP32)
     return oneshot(A, B, C, RVSP_SPGEMM_ALGO_RVV);
+  // The legacy dtypes (spgemm.c:182-196), on the GPU (legacy.cpp). The
+  // reference's vector variants of them need a RISC-V vector build.
+  if ((backend == RVSP_BACKEND_SCALAR || backend == RVSP_BACKEND_HIP) &&
+      in == RVSP_DTYPE_INT8 && out == RVSP_DTYPE_INT32)
+    return oneshot_legacy(A, B, C, true);
+  if ((backend == RVSP_BACKEND_SCALAR || backend == RVSP_BACKEND_HIP) &&
+      in == RVSP_DTYPE_FP64 && out == RVSP_DTYPE_FP64)
+    return oneshot_legacy(A, B, C, false);
   return RVSP_ERROR_UNSUPPORTED_BACKEND;
 }
 

@@ -140,6 +140,16 @@ rvsp_status_t ensure_map_built(rvsp_spgemm_descr* d, const DevCsr& A, const DevC
 rvsp_status_t promote_to_map(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
                              const int* c_ci, hipStream_t s);
 
+// rvsp_csr_check's verdict (and failing row, or -1) for device arrays,
+// computed on the GPU; key_buf is 8 bytes of scratch. Synchronous on s.
+rvsp_status_t csr_check_device(int rows, int cols, int nnz, const int* rp, const int* ci,
+                               DevBuf& key_buf, hipStream_t s, rvsp_csr_status_t* verdict,
+                               int* bad_row);
+// rvsp_spgemm_csr for FP64 x FP64 -> FP64 (i8 false) and INT8 x INT8 ->
+// INT32 (i8 true) operands (legacy.cpp).
+rvsp_status_t oneshot_legacy(const rvsp_csr_matrix_t* A, const rvsp_csr_matrix_t* B,
+                             rvsp_csr_matrix_t* C, bool i8);
+
 void destroy_graph(rvsp_spgemm_descr* d);
 void destroy_batch(rvsp_spgemm_descr* d);
 void invalidate_batch(rvsp_spgemm_descr* d);

@@ -141,6 +141,17 @@ hipError_t launch_scan(const int* cnt, int rows, int* c_rp, long long* total,
 // row_ptr[limit] (<= nnz when the row pass found nothing before limit).
 hipError_t launch_csr_check_rows(int rows, int nnz, const int* row_ptr, bool has_cols,
                                  unsigned long long* key, hipStream_t s);
+// FP64 (i8 false: double values, double sums) and INT8 -> INT32 (i8 true)
+// numeric phase over C's structure (c_rp, sorted c_ci): sums per position in
+// the reference's order into c_acc, non-zero sums per row into nonzero; then
+// the rows' non-zero entries to out_rp / out_ci / out_val (legacy.hip).
+hipError_t launch_numeric_legacy(bool i8, int rows, const int* a_rp, const int* a_ci,
+                                 const void* a_val, const int* b_rp, const int* b_ci,
+                                 const void* b_val, const int* c_rp, const int* c_ci,
+                                 void* c_acc, int* nonzero, hipStream_t s);
+hipError_t launch_compact_legacy(bool i8, int rows, const int* c_rp, const int* c_ci,
+                                 const void* c_acc, const int* out_rp, int* out_ci,
+                                 void* out_val, hipStream_t s);
 hipError_t launch_csr_check_cols(int limit, int cols, int nnz, const int* row_ptr,
                                  const int* col_idx, unsigned long long* key, hipStream_t s);
 size_t scan_tile_count(int rows);

@@ -13,15 +13,20 @@ from . import _ffi as F
 from ._ffi import check, lib
 
 
-class Csr:
-    """Host CSR matrix (int32 row_ptr/col_idx, float32 values)."""
+_DTYPES = {np.dtype(np.float32): F.DTYPE_FP32, np.dtype(np.float64): F.DTYPE_FP64,
+           np.dtype(np.int8): F.DTYPE_INT8, np.dtype(np.int32): F.DTYPE_INT32}
 
-    def __init__(self, rows, cols, row_ptr, col_idx, values):
+
+class Csr:
+    """Host CSR matrix (int32 row_ptr/col_idx; float32 values unless another
+    value dtype of the C API is given: float64, int8 or int32)."""
+
+    def __init__(self, rows, cols, row_ptr, col_idx, values, dtype=np.float32):
         self.rows = int(rows)
         self.cols = int(cols)
         self.row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
         self.col_idx = np.ascontiguousarray(col_idx, dtype=np.int32)
-        self.values = np.ascontiguousarray(values, dtype=np.float32)
+        self.values = np.ascontiguousarray(values, dtype=dtype)
 
     @property
     def nnz(self):
@@ -31,7 +36,8 @@ class Csr:
         """rvsp_csr_matrix_t viewing this object's arrays (keep self alive)."""
         return F.CsrMatrix(self.rows, self.cols, self.nnz,
                            self.row_ptr.ctypes.data, self.col_idx.ctypes.data,
-                           self.values.ctypes.data, F.DTYPE_FP32, F.FORMAT_CSR, 0)
+                           self.values.ctypes.data, _DTYPES[self.values.dtype],
+                           F.FORMAT_CSR, 0)
 
     @classmethod
     def from_dense(cls, dense):
@@ -253,6 +259,28 @@ def spgemm_oneshot(A, B, backend=F.BACKEND_SCALAR):
     finally:
         lib().rvsp_csr_destroy(C.byref(Cm))
     return Csr(A.rows, B.cols, rp, ci, v)
+
+
+def spgemm_legacy(A, B, backend=F.BACKEND_SCALAR):
+    """rvsp_spgemm_csr for float64 operands (FP64 -> FP64) or int8 operands
+    (INT8 -> INT32); returns C with float64 / int32 values."""
+    a, b = A.struct(), B.struct()
+    i8 = A.values.dtype == np.int8
+    dt_in, dt_out = (F.DTYPE_INT8, F.DTYPE_INT32) if i8 else (F.DTYPE_FP64, F.DTYPE_FP64)
+    Cm = F.CsrMatrix()
+    opts = F.SpgemmOptions(backend, dt_in, dt_out)
+    check(lib().rvsp_spgemm_csr(C.byref(a), C.byref(b), C.byref(Cm), C.byref(opts)),
+          "spgemm_csr")
+    try:
+        nnz = Cm.nnz
+        vt = C.c_int32 if i8 else C.c_double
+        rp = np.ctypeslib.as_array((C.c_int32 * (Cm.rows + 1)).from_address(Cm.row_ptr)).copy()
+        ci = np.ctypeslib.as_array((C.c_int32 * max(nnz, 1)).from_address(Cm.col_idx))[:nnz].copy()
+        v = np.ctypeslib.as_array((vt * max(nnz, 1)).from_address(Cm.values))[:nnz].copy()
+        assert Cm.dtype == dt_out
+    finally:
+        lib().rvsp_csr_destroy(C.byref(Cm))
+    return Csr(A.rows, B.cols, rp, ci, v, dtype=np.int32 if i8 else np.float64)
 
 
 def spgemm_device(A, B, algo=F.ALGO_DEFAULT, choices=None, repeats=1):

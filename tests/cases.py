@@ -178,3 +178,31 @@ def csr_check_cases(n_seeds=12):
     out.append(("first_col_negative", 2, 4, np.array([0, 1, 2], np.int32),
                 np.array([-1, 0], np.int32)))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Legacy-dtype products (FP64 -> FP64, INT8 -> INT32): seeded canonical
+# operands. Values come from small integer sets so that many sums cancel to
+# exactly zero (those entries are dropped); FP64 cases also carry fractional
+# values, where the order of the additions shows in the last bits.
+
+def legacy_pair(seed, i8, rows=120, inner=90, cols=3000, mean=4.0, long_rows=2):
+    rng = np.random.Generator(np.random.PCG64(5000 + seed))
+    la = np.minimum(rng.poisson(mean, size=rows), inner)
+    la[rng.choice(rows, size=min(long_rows, rows), replace=False)] = inner  # wide C rows
+    lb = np.minimum(rng.poisson(mean * 4, size=inner), cols)
+    lb[rng.random(inner) < 0.1] = 0
+    rpa, cia = gen.random_rows(rows, inner, la, rng)
+    rpb, cib = gen.random_rows(inner, cols, lb, rng)
+    if i8:
+        va = rng.integers(-128, 128, size=len(cia)).astype(np.int8)
+        vb = rng.choice(np.array([-2, -1, 1, 2], np.int8), size=len(cib))
+        dt = np.int8
+    else:
+        va = rng.integers(-2, 3, size=len(cia)).astype(np.float64)
+        va[va == 0] = 1.0
+        vb = rng.choice(np.array([-1.0, 1.0, 0.5, -0.5]), size=len(cib))
+        frac = rng.random(len(cib)) < 0.3
+        vb[frac] = rng.standard_normal(int(frac.sum()))
+        dt = np.float64
+    return (Csr(rows, inner, rpa, cia, va, dtype=dt), Csr(inner, cols, rpb, cib, vb, dtype=dt))

@@ -179,7 +179,12 @@ def test_oneshot_rejects_before_device_work():
     B = Csr(2, 2, [0, 1, 2], [0, 1], [4, 5])
     a, b, Cm = A.struct(), B.struct(), F.CsrMatrix()
     assert L.rvsp_spgemm_csr(C.byref(a), C.byref(b), C.byref(Cm), None) == F.ERROR_INVALID_CSR
+    # FP64 options reach the FP64 wrapper, which refuses FP32 operands
+    # (csr_spgemm_wrappers.c:362-365) before any device work.
     opts = F.SpgemmOptions(F.BACKEND_SCALAR, F.DTYPE_FP64, F.DTYPE_FP64)
+    assert L.rvsp_spgemm_csr(C.byref(a), C.byref(b), C.byref(Cm),
+                             C.byref(opts)) == F.ERROR_INVALID_ARGUMENT
+    opts = F.SpgemmOptions(F.BACKEND_SCALAR, F.DTYPE_FP64, F.DTYPE_FP32)
     assert L.rvsp_spgemm_csr(C.byref(a), C.byref(b), C.byref(Cm),
                              C.byref(opts)) == F.ERROR_UNSUPPORTED_BACKEND
     assert L.rvsp_spgemm_csr(None, C.byref(b), C.byref(Cm), None) == F.ERROR_NULL_POINTER

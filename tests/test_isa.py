@@ -31,7 +31,8 @@ def device_asm(src):
 def test_no_fma_no_packed_fp32(src):
     asm = device_asm(os.path.join("csrc", src))
     assert "v_add_f32" in asm or "v_mul_f32" in asm or "kernels" not in src
-    fused = re.findall(r"\b(v_fma_f32|v_fmac_f32\w*|v_fma_mix\w*|v_mad_f32|v_mac_f32)\b", asm)
+    fused = re.findall(r"\b(v_fma_f(?:32|64)|v_fmac_f(?:32|64)\w*|v_fma_mix\w*|v_mad_f32|"
+                       r"v_mac_f32)\b", asm)
     assert not fused, f"fused multiply-add in device code: {sorted(set(fused))}"
     packed = re.findall(r"\bv_pk_(?:add|mul|fma)_f32\b", asm)
     assert not packed, f"packed FP32 instructions in device code: {sorted(set(packed))}"
