@@ -545,6 +545,49 @@ def bench_banded(hip, args, F):
     }
 
 
+def bench_legacy(hip, args, F):
+    """FP64 -> FP64 and INT8 -> INT32 one-shot products (the reference's
+    legacy kernels) of the 5-point Laplacian with itself through
+    rvsp_spgemm_csr on host arrays: the GPU call end to end (copies in and
+    out included) against the reference's own scalar legacy kernel, or the
+    oracle's restatement, on one host core."""
+    import rvsp
+    from oracle import oracle
+    from rvsp import gen
+    from rvsp.api import Csr
+    A = gen.laplacian_2d(args.grid)
+    products = int(np.diff(A.row_ptr)[A.col_idx].sum())
+    rng = np.random.default_rng(0)
+    out = {"workload": f"laplacian5pt_g{args.grid}_AxA_oneshot_legacy_dtypes", "products": products}
+    for name, vals, dt in (
+            ("fp64", rng.uniform(0.5, 1.5, size=A.nnz), np.float64),
+            ("int8", rng.integers(-128, 128, size=A.nnz), np.int8)):
+        M = Csr(A.rows, A.cols, A.row_ptr, A.col_idx, vals, dtype=dt)
+        rvsp.spgemm_legacy(M, M)  # warm-up
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            got = rvsp.spgemm_legacy(M, M)
+            times.append(time.perf_counter() - t0)
+        t_gpu = statistics.median(times)
+        kind = "reference" if oracle.ref_available() else "port"
+        t0 = time.perf_counter()
+        if kind == "reference":
+            st, want = oracle.ref_spgemm_legacy(M, M)
+        else:
+            want = oracle.spgemm_legacy(M, M)
+        t_cpu = time.perf_counter() - t0
+        same = all(g.tobytes() == w.tobytes()
+                   for g, w in zip((got.row_ptr, got.col_idx, got.values), want))
+        out[name] = {"oneshot_ms": round(t_gpu * 1e3, 2), "nnz_c": int(got.nnz),
+                     "gflops": round(2.0 * products / t_gpu / 1e9, 2),
+                     "cpu_ms": round(t_cpu * 1e3, 1), "cpu_kind": kind, "cpu_cores": 1,
+                     "speedup_vs_cpu": round(t_cpu / t_gpu, 2), "verified_bit_exact": bool(same)}
+        print(f"# legacy {name}: GPU one-shot {t_gpu * 1e3:.2f} ms, CPU ({kind}) "
+              f"{t_cpu * 1e3:.1f} ms, bit-exact {same}", file=sys.stderr, flush=True)
+    return out
+
+
 def bench_headline(hip, args, F):
     """The headline: numeric-only multiplies of the Laplacian product."""
     import rvsp
@@ -667,7 +710,8 @@ def main():
     ap.add_argument("--only", choices=["fixed", "tuned"], default=None,
                     help="time one plan only (profiling runs)")
     ap.add_argument("--extra", default="batched,powerlaw,banded",
-                    help="secondary workloads reported under 'configs' (comma list, or none)")
+                    help="secondary workloads reported under 'configs' (comma list from "
+                         "batched, powerlaw, banded, legacy; or none)")
     ap.add_argument("--only-extra", action="store_true",
                     help="skip the headline workload (profiling runs of the extras)")
     ap.add_argument("--pairs", type=int, default=2048)
@@ -705,7 +749,8 @@ def main():
 
     hip = Hip()
     out = {} if args.only_extra else bench_headline(hip, args, F)
-    extras = {"batched": bench_batched, "powerlaw": bench_powerlaw, "banded": bench_banded}
+    extras = {"batched": bench_batched, "powerlaw": bench_powerlaw, "banded": bench_banded,
+              "legacy": bench_legacy}
     configs = {}
     for name in extra:
         t0 = time.perf_counter()
