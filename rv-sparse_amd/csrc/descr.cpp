@@ -283,6 +283,30 @@ rvsp_status_t fetch_offsets(rvsp_spgemm_descr* d, PhasePlan* plans[], int n,
 
 int bin_rows(const PhasePlan& p, int b) { return p.offsets[b + 1] - p.offsets[b]; }
 
+// Finds the bins whose rows are one run of consecutive rows (row lists are
+// ascending and distinct, so first and last decide). Synchronous.
+rvsp_status_t fetch_row_bases(PhasePlan& p, hipStream_t s) {
+  const int nb = p.table.rule.nbins;
+  p.row_base.assign(nb, -1);
+  std::vector<int> ends(2 * (size_t)nb, 0);
+  bool any = false;
+  for (int b = 1; b < nb; ++b) {
+    const int n = bin_rows(p, b);
+    if (n == 0) continue;
+    const int* r = (const int*)p.rows.p + p.offsets[b];
+    RVSP_HIP_TRY(hipMemcpyAsync(&ends[2 * b], r, 4, hipMemcpyDeviceToHost, s));
+    RVSP_HIP_TRY(hipMemcpyAsync(&ends[2 * b + 1], r + n - 1, 4, hipMemcpyDeviceToHost, s));
+    any = true;
+  }
+  if (!any) return RVSP_SUCCESS;
+  RVSP_HIP_TRY(hipStreamSynchronize(s));
+  for (int b = 1; b < nb; ++b) {
+    const int n = bin_rows(p, b);
+    if (n > 0 && ends[2 * b + 1] - ends[2 * b] == n - 1) p.row_base[b] = ends[2 * b];
+  }
+  return RVSP_SUCCESS;
+}
+
 // Sizes the global scratch slots a phase's GLOBAL bins need. Bitmaps are
 // cleared once at allocation; the kernels leave them cleared.
 rvsp_status_t ensure_slots(rvsp_spgemm_descr* d, const PhasePlan& p, int b_cols,
@@ -449,6 +473,7 @@ rvsp_status_t estimate(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
   RVSP_TRY(partition_phase(d, d->numeric, M, 2, s));
   PhasePlan* fp[] = {&d->fill, &d->numeric};
   RVSP_TRY(fetch_offsets(d, fp, 2, 1, s));
+  RVSP_TRY(fetch_row_bases(d->numeric, s));
   RVSP_TRY(select_phase(d, d->fill, PHASE_FILL, A, (const int*)d->c_rp.p, s));
   RVSP_TRY(select_phase(d, d->numeric, PHASE_NUMERIC, A, (const int*)d->c_rp.p, s));
   RVSP_TRY(plan_map(d, s));
@@ -545,6 +570,7 @@ rvsp_status_t run_numeric(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B
       mv.map = d->map.p;
       mv.poff = (const long long*)d->poff.p;
       mv.width = d->map_width;
+      mv.row_base = numeric_row_base(d, rows);
       RVSP_HIP_TRY(launch_numeric_map_bin(spec, mv, rows, n, A, B, (const int*)d->c_rp.p,
                                           c_val, s));
     } else {
@@ -554,6 +580,14 @@ rvsp_status_t run_numeric(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B
     }
   }
   return RVSP_SUCCESS;
+}
+
+int numeric_row_base(const rvsp_spgemm_descr* d, const int* rows) {
+  const PhasePlan& p = d->numeric;
+  for (size_t b = 1; b < p.row_base.size() && b < p.offsets.size(); ++b)
+    if ((const int*)p.rows.p + p.offsets[b] == rows && bin_rows(p, (int)b) > 0)
+      return p.row_base[b];
+  return -1;
 }
 
 rvsp_status_t plan_map(rvsp_spgemm_descr* d, hipStream_t s) {

@@ -30,6 +30,9 @@ struct PhasePlan {
   PhaseTable table;
   std::vector<int> offsets;  // host copy, nbins + 1 entries
   DevBuf rows;               // row ids grouped by bin, ascending within a bin
+  // Per bin: r when the bin's rows are exactly r, r + 1, ... (kernels that
+  // can use it then skip reading the row list), else -1.
+  std::vector<int> row_base;
 };
 
 }  // namespace rvsp
@@ -149,6 +152,9 @@ rvsp_status_t csr_check_device(int rows, int cols, int nnz, const int* rp, const
 // INT32 (i8 true) operands (legacy.cpp).
 rvsp_status_t oneshot_legacy(const rvsp_csr_matrix_t* A, const rvsp_csr_matrix_t* B,
                              rvsp_csr_matrix_t* C, bool i8);
+
+// row_base of the numeric bin whose row list starts at `rows` (-1 if none).
+int numeric_row_base(const rvsp_spgemm_descr* d, const int* rows);
 
 void destroy_graph(rvsp_spgemm_descr* d);
 void destroy_batch(rvsp_spgemm_descr* d);

@@ -1251,7 +1251,7 @@ hipError_t launch_build_map(int width, const int* rows, int nrows, const DevCsr&
 // (VAR 1). VAR 2 instead fetches A entries in chunks (see below).
 template <int G, typename T, int VAR>
 __global__ __launch_bounds__(kBlock) void k_numeric_map(
-    const int* __restrict__ rows, int nrows, int cap, DevCsr A, DevCsr B,
+    const int* __restrict__ rows, int nrows, int row_base, int cap, DevCsr A, DevCsr B,
     const int* __restrict__ c_rp, const long long* __restrict__ poff,
     const T* __restrict__ map, float* __restrict__ c_val) {
   extern __shared__ int lds[];
@@ -1272,8 +1272,9 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
   const int g = threadIdx.x / G;
   const int l = threadIdx.x & (G - 1);
   const bool active = g < nr;
-  const int first = rows[r0];
-  const int last = rows[r0 + nr - 1];
+  // A bin of consecutive rows needs no look into its row list.
+  const int first = row_base >= 0 ? row_base + r0 : rows[r0];
+  const int last = row_base >= 0 ? first + nr - 1 : rows[r0 + nr - 1];
   const bool seg = (last - first == nr - 1);
 
   int c_s = 0, n = 0, a_s = 0, a_e = 0;
@@ -1514,14 +1515,14 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
 
 template <int G>
 static hipError_t launch_numeric_map_g(const BinSpec& b, int width, const int* rows, int nrows,
-                                       const DevCsr& A, const DevCsr& B, const int* c_rp,
-                                       const long long* poff, const void* map, float* c_val,
-                                       hipStream_t s) {
+                                       int row_base, const DevCsr& A, const DevCsr& B,
+                                       const int* c_rp, const long long* poff, const void* map,
+                                       float* c_val, hipStream_t s) {
   const int groups = kBlock / G;
   const int grid = (nrows + groups - 1) / groups;
 #define RVSP_MAP_LAUNCH(T, SA)                                                              \
   hipLaunchKernelGGL((k_numeric_map<G, T, SA>), dim3(grid), dim3(kBlock), b.lds_bytes, s, rows, \
-                     nrows, b.cap, A, B, c_rp, poff, (const T*)map, c_val)
+                     nrows, row_base, b.cap, A, B, c_rp, poff, (const T*)map, c_val)
   if (b.variant < 0 || b.variant > 2) return hipErrorInvalidValue;
   if (width == 1) {
     if (b.variant == 0) RVSP_MAP_LAUNCH(unsigned char, 0);
@@ -1542,13 +1543,13 @@ hipError_t launch_numeric_map_bin(const BinSpec& b, const MapView& mv, const int
   if (nrows <= 0) return hipSuccess;
   if (!mv.map || !mv.poff || (mv.width != 1 && mv.width != 2)) return hipErrorInvalidValue;
   switch (b.group) {
-    case 1: return launch_numeric_map_g<1>(b, mv.width, rows, nrows, A, B, c_rp, mv.poff, mv.map, c_val, s);
-    case 2: return launch_numeric_map_g<2>(b, mv.width, rows, nrows, A, B, c_rp, mv.poff, mv.map, c_val, s);
-    case 4: return launch_numeric_map_g<4>(b, mv.width, rows, nrows, A, B, c_rp, mv.poff, mv.map, c_val, s);
-    case 8: return launch_numeric_map_g<8>(b, mv.width, rows, nrows, A, B, c_rp, mv.poff, mv.map, c_val, s);
-    case 16: return launch_numeric_map_g<16>(b, mv.width, rows, nrows, A, B, c_rp, mv.poff, mv.map, c_val, s);
-    case 32: return launch_numeric_map_g<32>(b, mv.width, rows, nrows, A, B, c_rp, mv.poff, mv.map, c_val, s);
-    case 64: return launch_numeric_map_g<64>(b, mv.width, rows, nrows, A, B, c_rp, mv.poff, mv.map, c_val, s);
+    case 1: return launch_numeric_map_g<1>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, s);
+    case 2: return launch_numeric_map_g<2>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, s);
+    case 4: return launch_numeric_map_g<4>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, s);
+    case 8: return launch_numeric_map_g<8>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, s);
+    case 16: return launch_numeric_map_g<16>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, s);
+    case 32: return launch_numeric_map_g<32>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, s);
+    case 64: return launch_numeric_map_g<64>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, s);
     default: return hipErrorInvalidValue;
   }
 }

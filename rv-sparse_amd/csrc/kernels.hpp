@@ -201,6 +201,7 @@ struct MapView {
   const void* map = nullptr;        // uint8 or uint16 entries
   const long long* poff = nullptr;  // rows + 1 entries
   int width = 0;                    // bytes per entry (1 or 2); 0 = no map
+  int row_base = -1;  // >= 0: the listed rows are row_base, row_base + 1, ...
 };
 // poff = exclusive scan over rows of (bin_of[i] in bin_mask ? Op_i : 0);
 // *total = poff[rows]. `weights` holds rows int64 values of scratch.

@@ -159,6 +159,7 @@ hipError_t launch_bin(rvsp_spgemm_descr* d, int phase, const BinSpec& spec, cons
         mv.map = d->map.p;
         mv.poff = (const long long*)d->poff.p;
         mv.width = d->map_width;
+        mv.row_base = numeric_row_base(d, rows);
         return launch_numeric_map_bin(spec, mv, rows, n, A, B, (const int*)d->c_rp.p, c_val, s);
       }
       if (spec.kind == KIND_BLOCK && ensure_split(d, B, s) != RVSP_SUCCESS)
