@@ -501,6 +501,8 @@ rvsp_status_t rvsp_spgemm_batch_work_estimation(rvsp_spgemm_descr_t descr, int32
   // counts, so the lanes' host round trips and kernels overlap). Different
   // descriptors are independent; the library's shared tables are locked.
   const int lanes = bs.groups.size() > 1 && bs.lanes_ready() ? BatchState::kLanes : 1;
+  if (lanes > 1)
+    for (BatchGroup& g : bs.groups) g.inner->shared_cu = true;
   auto estimate_lane = [&](int t) -> rvsp_status_t {
     hipStream_t ls = t ? bs.lane[t - 1] : s;
     DevBuf& gidx = bs.gather_idx[t];

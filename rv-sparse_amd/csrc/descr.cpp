@@ -475,6 +475,16 @@ rvsp_status_t estimate(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
   RVSP_TRY(fetch_offsets(d, fp, 2, 1, s));
   RVSP_TRY(fetch_row_bases(d->numeric, s));
   RVSP_TRY(select_phase(d, d->fill, PHASE_FILL, A, (const int*)d->c_rp.p, s));
+  if (d->shared_cu)
+    for (int b = 2; b < d->fill.table.rule.nbins; ++b) {
+      BinSpec& f = d->fill.table.spec[b];
+      if (f.kind == KIND_BITMAP && f.lds_bytes > kWideBitmapBytes) {
+        f.kind = KIND_GLOBAL;
+        f.group = kBlock;
+        f.cap = 0;
+        f.lds_bytes = 0;
+      }
+    }
   RVSP_TRY(select_phase(d, d->numeric, PHASE_NUMERIC, A, (const int*)d->c_rp.p, s));
   RVSP_TRY(plan_map(d, s));
   RVSP_TRY(ensure_slots(d, d->fill, B.cols, false, s));

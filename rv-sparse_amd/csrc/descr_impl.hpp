@@ -104,6 +104,12 @@ struct rvsp_spgemm_descr {
   bool batch_graph = true;
   long long batch_limit = 0;  // products per group; 0 = INT32_MAX
 
+  // Set on a batch's inner descriptors whose groups run on two streams at
+  // once: fill bins planned as a wide LDS bitmap (one workgroup per CU, so
+  // it waits for a whole CU to drain of the other stream's workgroups) run
+  // their bitmaps in global scratch instead and share CUs.
+  bool shared_cu = false;
+
   // Set by the one-shot path: work_estimation checks the staged operands
   // for canonical CSR on the device before any other kernel reads them.
   bool check_csr = false;

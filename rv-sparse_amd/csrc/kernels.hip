@@ -493,7 +493,6 @@ __global__ __launch_bounds__(kBlock) void k_count_hash(
 // phase copies them instead of building the bitmap again.
 constexpr int kBitmapCount = 0, kBitmapFill = 1, kBitmapCountKeep = 2;
 constexpr int kWideBlock = 1024;              // threads of a workgroup holding a large bitmap
-constexpr int kWideBitmapBytes = 80 * 1024;   // bitmaps above this fit once per CU
 
 // One workgroup per row. The row's columns lie in [lo, hi]; a bitmap over that
 // span lives in LDS (GLOBAL_BM == false, span <= 32 * words) or in this

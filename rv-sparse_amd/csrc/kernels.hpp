@@ -131,6 +131,10 @@ hipError_t launch_copy_kept(const int* rows, int nrows, const int* keep,
                             hipStream_t s);
 hipError_t launch_scan(const int* cnt, int rows, int* c_rp, long long* total,
                        long long* tile_sums, hipStream_t s);
+// LDS bitmaps above this size fit once per CU; their workgroups get 1024
+// threads.
+constexpr int kWideBitmapBytes = 80 * 1024;
+
 // Canonical-CSR check of device arrays (reference rvsp_support.c:36-126).
 // Failures are keyed (row << 32) | code and *key keeps the smallest (the
 // caller sets it to ~0 first): code 0 a row pointer that decreases or (with

@@ -93,6 +93,35 @@ def test_row_blocks_in_groups_share_the_split_table(gpu):
         assert g.values.tobytes() == v[s:e].tobytes()
 
 
+def test_concurrent_groups_with_wide_fill_bitmaps(gpu):
+    """Row blocks of an R-MAT 2^20 product in two groups (two streams): rows
+    whose fill a single descriptor plans as an LDS bitmap over 2^20 columns
+    (one workgroup per CU) are filled from global scratch in the batch; every
+    block still equals the oracle's rows."""
+    A = gen.rmat(20, seed=0)
+    cuts = [(400000, 401500), (700000, 701500)]
+    blocks = []
+    for r0, r1 in cuts:
+        s, e = A.row_ptr[r0], A.row_ptr[r1]
+        blocks.append(Csr(r1 - r0, A.cols, A.row_ptr[r0:r1 + 1] - s, A.col_idx[s:e],
+                          A.values[s:e]))
+    # A single descriptor plans a wide LDS-bitmap fill bin for these rows.
+    d = rvsp.Descriptor(algo=F.ALGO_HIP_FIXED)
+    d.estimate(blocks[0], A)
+    wide = [b for b in d.plan() if b["phase"] == 1 and b["kind"] == "lds_bitmap" and
+            b["rows"] > 0 and b["lds_bytes"] > 80 * 1024]
+    d.close()
+    assert wide, "precondition: a wide LDS-bitmap fill bin"
+    ops = [int(np.diff(A.row_ptr)[blk.col_idx].sum()) for blk in blocks]
+    got, groups = spgemm_batch([(blk, A) for blk in blocks], group_limit=max(ops))
+    assert groups == 2
+    for (r0, r1), g in zip(cuts, got):
+        rp, ci, v = oracle.spgemm(A, A, r0, r1)
+        assert np.array_equal(g.row_ptr, rp)
+        assert g.col_idx.tobytes() == ci.tobytes()
+        assert g.values.tobytes() == v.tobytes()
+
+
 @pytest.mark.parametrize("graph", [True, False])
 def test_repeated_numeric_computes_with_new_values(gpu, graph):
     pairs = small_pairs(count=12, seed=2)
