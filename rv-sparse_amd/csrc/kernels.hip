@@ -526,10 +526,17 @@ __global__ __launch_bounds__(NT) void k_bitmap_row(
     block_for_each_col<NT>(A.rp[row], A.rp[row + 1], A.ci, B.rp, B.ci, sh_excl,
                        sh_bs, sh_etab, tmp, [&](int col, bool valid) {
                          if (valid) {
+                           // Bits are only ever set: a bit seen set needs no
+                           // atomic, which spares the serialised atomics
+                           // of the many products that land on columns
+                           // already seen (hub columns in every B row).
                            const int d = col - base_col;
-                           __hip_atomic_fetch_or(&bits[d >> 5], 1u << (d & 31),
-                                                 __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+                           unsigned* w = &bits[d >> 5];
+                           const unsigned m = 1u << (d & 31);
+                           if (!(__hip_atomic_load(w, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP) & m))
+                             __hip_atomic_fetch_or(w, m, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
                          }
                        });
     // block_for_each_col ends with a barrier, which also orders the global
