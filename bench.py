@@ -732,9 +732,9 @@ def main():
         print(json.dumps({"calibration_copy_bytes": n, "copies": 3}))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has one MI355X, and "
-              "the SpGEMM path has no multi-GPU mode (DESIGN.md describes how it would "
-              "shard by output rows).", file=sys.stderr)
+        print("bench.py measures one GPU: the SpGEMM path has no multi-GPU mode. It "
+              "would shard by output rows: each GPU runs this pipeline on a block of "
+              "A's rows balanced by flop count, with a replica of B.", file=sys.stderr)
         sys.exit(2)
 
     extra = [x for x in args.extra.split(",") if x and x != "none"]
