@@ -1687,6 +1687,7 @@ __device__ __forceinline__ int gm_gallop(const int* __restrict__ b_ci, int lo, i
 // kBlock entries), and split tables bound the searches of long B rows.
 constexpr int kTiledItems = 2;             // products per lane per group
 constexpr int kTiledGroup = kTiledItems * kBlock;
+constexpr int kTiledDepth = 2;             // groups whose gathers are in flight
 constexpr int kTiledWaves = kBlock / kWave;
 constexpr int kTagSlots = 256;             // per-wave hash of pending positions
 constexpr int kEntrySlots = 1024;          // entry table of a chunk
@@ -1885,46 +1886,50 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
         }
         __syncthreads();
         // Groups of kTiledGroup products, kTiledItems per lane (products
-        // q0 + i kBlock + tid), software-pipelined: the next group's gathers
-        // are issued before this group's lookups and adds.
-        int n_col[kTiledItems];
-        float n_b[kTiledItems], n_a[kTiledItems];
-#pragma unroll
-        for (int i = 0; i < kTiledItems; ++i) {
-          const int q = i * kBlock + tid;
-          n_col[i] = -1;
-          n_b[i] = n_a[i] = 0.0f;
+        // q0 + i kBlock + tid), software-pipelined: the gathers of the next
+        // kTiledDepth groups are issued before this group's lookups and adds
+        // (occupancy is bounded by LDS here, not registers).
+        int n_col[kTiledDepth][kTiledItems];
+        float n_b[kTiledDepth][kTiledItems], n_a[kTiledDepth][kTiledItems];
+        auto gather = [&](int q, int& c, float& bv_, float& av_) {
+          // The A entry of product q (entries without products share their
+          // successor's prefix).
+          c = -1;
+          bv_ = av_ = 0.0f;
           if (q < total) {
             const int e = tiled_entry(L, shift, q);
             const int bi = L.bstart[e] + (q - L.pre[e]);
-            n_col[i] = B.ci[bi];
-            n_b[i] = B.val[bi];
-            n_a[i] = L.aval[e];
+            c = B.ci[bi];
+            bv_ = B.val[bi];
+            av_ = L.aval[e];
           }
-        }
+        };
+#pragma unroll
+        for (int d = 0; d < kTiledDepth; ++d)
+#pragma unroll
+          for (int i = 0; i < kTiledItems; ++i)
+            gather(d * kTiledGroup + i * kBlock + tid, n_col[d][i], n_b[d][i], n_a[d][i]);
         for (int q0 = 0; q0 < total; q0 += kTiledGroup) {
           int col[kTiledItems];
           float bv[kTiledItems], av[kTiledItems];
 #pragma unroll
           for (int i = 0; i < kTiledItems; ++i) {
-            col[i] = n_col[i];
-            bv[i] = n_b[i];
-            av[i] = n_a[i];
+            col[i] = n_col[0][i];
+            bv[i] = n_b[0][i];
+            av[i] = n_a[0][i];
           }
 #pragma unroll
-          for (int i = 0; i < kTiledItems; ++i) {
-            // The A entry of product qn (entries without products share
-            // their successor's prefix).
-            const int qn = q0 + (kTiledItems + i) * kBlock + tid;
-            n_col[i] = -1;
-            if (qn < total) {
-              const int e = tiled_entry(L, shift, qn);
-              const int bi = L.bstart[e] + (qn - L.pre[e]);
-              n_col[i] = B.ci[bi];
-              n_b[i] = B.val[bi];
-              n_a[i] = L.aval[e];
+          for (int d = 0; d + 1 < kTiledDepth; ++d)
+#pragma unroll
+            for (int i = 0; i < kTiledItems; ++i) {
+              n_col[d][i] = n_col[d + 1][i];
+              n_b[d][i] = n_b[d + 1][i];
+              n_a[d][i] = n_a[d + 1][i];
             }
-          }
+#pragma unroll
+          for (int i = 0; i < kTiledItems; ++i)
+            gather(q0 + kTiledDepth * kTiledGroup + i * kBlock + tid, n_col[kTiledDepth - 1][i],
+                   n_b[kTiledDepth - 1][i], n_a[kTiledDepth - 1][i]);
           int pos[kTiledItems], own[kTiledItems], rank[kTiledItems];
           float prod[kTiledItems];
 #pragma unroll
