@@ -1721,7 +1721,7 @@ __device__ __forceinline__ TiledLds tiled_lds(int* lds, int T) {
   return L;
 }
 
-constexpr int kCurChunks = 4;
+constexpr int kCurChunks = 16;
 
 // The A entry of product q of a chunk (the last e with pre[e] <= q): the
 // entry table gives the entry of the slot's first product, the kPrePad next
