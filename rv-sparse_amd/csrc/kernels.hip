@@ -2147,7 +2147,17 @@ __global__ __launch_bounds__(kBlock) void k_copy_kept(const int* __restrict__ ro
     const int* src = keep + koff[row];
     const int per = (n + gridDim.y - 1) / gridDim.y;
     const int j0 = blockIdx.y * per, j1 = min(n, j0 + per);
-    for (int j = j0 + threadIdx.x; j < j1; j += kBlock) c_ci[c_s + j] = src[j];
+    // Four loads in flight per thread before their stores.
+    constexpr int kU = 4;
+    int j = j0 + threadIdx.x;
+    for (; j + (kU - 1) * kBlock < j1; j += kU * kBlock) {
+      int v[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) v[u] = src[j + u * kBlock];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) c_ci[c_s + j + u * kBlock] = v[u];
+    }
+    for (; j < j1; j += kBlock) c_ci[c_s + j] = src[j];
   }
 }
 
