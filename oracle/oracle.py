@@ -348,6 +348,71 @@ def time_reference_numeric(A, B, reps, struct_cls, values_list=None):
     return times, nnz.value
 
 
+def time_reference_numeric_threads(A, B, threads, seconds, struct_cls):
+    """All-cores figure of the reference's numeric-only compute: A's rows in
+    `threads` contiguous blocks balanced by products, one reference
+    descriptor per block (descriptors are independent; the library keeps no
+    global state), each block's computes repeated on its own host thread for
+    about `seconds`. Returns (products per second, threads)."""
+    import threading
+    import time
+    h = _ref_descr_api()
+    lens = np.diff(B.row_ptr).astype(np.int64)
+    cs = np.concatenate([[0], np.cumsum(lens[A.col_idx])])
+    cum = np.concatenate([[0], np.cumsum(cs[A.row_ptr[1:]] - cs[A.row_ptr[:-1]])])
+    cuts = [int(np.searchsorted(cum, cum[-1] * t / threads)) for t in range(threads)] + [A.rows]
+    b = struct_cls(B.rows, B.cols, B.nnz, B.row_ptr.ctypes.data, B.col_idx.ctypes.data,
+                   B.values.ctypes.data, 3, 0, 0)
+    work = []
+    for r0, r1 in zip(cuts[:-1], cuts[1:]):
+        if r1 <= r0:
+            continue
+        o0, o1 = int(A.row_ptr[r0]), int(A.row_ptr[r1])
+        rp = np.ascontiguousarray(A.row_ptr[r0:r1 + 1] - o0, dtype=np.int32)
+        ci = np.ascontiguousarray(A.col_idx[o0:o1])
+        va = np.ascontiguousarray(A.values[o0:o1])
+        a = struct_cls(r1 - r0, A.cols, o1 - o0, rp.ctypes.data, ci.ctypes.data, va.ctypes.data,
+                       3, 0, 0)
+        d = C.c_void_p()
+        h.rvsp_spgemm_descr_create(C.byref(d))
+        ws_bytes, nnz = C.c_size_t(), _I32()
+        if h.rvsp_spgemm_work_estimation(d, C.byref(a), C.byref(b), C.byref(ws_bytes),
+                                         C.byref(nnz)):
+            raise RuntimeError("reference work_estimation failed")
+        n = max(nnz.value, 1)
+        out = (np.zeros(r1 - r0 + 1, np.int32), np.zeros(n, np.int32), np.zeros(n, np.float32))
+        cm = struct_cls(r1 - r0, B.cols, nnz.value, out[0].ctypes.data, out[1].ctypes.data,
+                        out[2].ctypes.data, 3, 0, 0)
+        ws = np.zeros(ws_bytes.value, np.uint8)
+        h.rvsp_spgemm_compute(d, C.byref(a), C.byref(b), C.byref(cm), _ptr(ws))  # fill
+        work.append({"d": d, "a": a, "cm": cm, "ws": ws, "keep": (rp, ci, va, out),
+                     "ops": int(cum[r1] - cum[r0]), "reps": 0})
+    start = threading.Barrier(len(work) + 1)
+    stop_at = [0.0]
+
+    def run(w):
+        start.wait()
+        while time.perf_counter() < stop_at[0] or w["reps"] == 0:
+            h.rvsp_spgemm_compute(w["d"], C.byref(w["a"]), C.byref(b), C.byref(w["cm"]),
+                                  _ptr(w["ws"]))
+            w["reps"] += 1
+        w["end"] = time.perf_counter()
+
+    ts = [threading.Thread(target=run, args=(w,)) for w in work]
+    for t in ts:
+        t.start()
+    t0 = time.perf_counter()
+    stop_at[0] = t0 + seconds
+    start.wait()
+    for t in ts:
+        t.join()
+    wall = max(w["end"] for w in work) - t0
+    ops = sum(w["ops"] * w["reps"] for w in work)
+    for w in work:
+        h.rvsp_spgemm_descr_destroy(w["d"])
+    return ops / wall, len(work)
+
+
 def time_port_numeric(A, B, reps):
     """Seconds per numeric pass of this project's oracle restatement."""
     import time
