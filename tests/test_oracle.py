@@ -161,3 +161,14 @@ def test_oracle_row_block_equals_full():
     assert np.concatenate([rp1, rp2[1:] + rp1[-1]]).tolist() == rp.tolist()
     assert np.concatenate([ci1, ci2]).tobytes() == ci.tobytes()
     assert np.concatenate([v1, v2]).tobytes() == val.tobytes()
+
+
+@ref
+def test_threaded_reference_timing_runs_every_block():
+    # The bench's all-cores CPU figure: the reference's descriptors on
+    # product-balanced row blocks, one thread each, all finishing.
+    from rvsp import _ffi as F
+    A = gen.laplacian_2d(64)
+    A = A.with_values(gen.laplacian_values(A, 0))
+    rate, used = oracle.time_reference_numeric_threads(A, A, 3, 0.05, F.CsrMatrix)
+    assert used == 3 and rate > 0
