@@ -29,20 +29,36 @@ rvsp_status_t ensure(DevBuf& b, size_t bytes) {
   if (bytes == 0) bytes = 4;
   if (b.bytes >= bytes) return RVSP_SUCCESS;
   release(b);
-  hipError_t e = hipMalloc(&b.p, bytes);
+  size_t got = bytes;
+  hipError_t e = pool_alloc(&b.p, &got);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     b.p = nullptr;
     return from_hip(e);
   }
-  b.bytes = bytes;
+  b.bytes = got;
   return RVSP_SUCCESS;
 }
 
 void release(DevBuf& b) {
-  if (b.p) (void)hipFree(b.p);
+  pool_free(b.p, b.bytes, false);
   b.p = nullptr;
   b.bytes = 0;
+}
+
+void release_idle(DevBuf& b) {
+  pool_free(b.p, b.bytes, true);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+size_t device_free_bytes() {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    free_b = 0;
+  }
+  return free_b + pool_idle_bytes();
 }
 
 rvsp_status_t device_ready() {
@@ -359,7 +375,7 @@ rvsp_status_t select_phase(rvsp_spgemm_descr* d, PhasePlan& p, int phase, const 
     std::vector<BinFeatures> f;
     RVSP_TRY(bin_features(d, p, phase, A.rp, c_rp, s, &f));
     if (tuned) select_tuned(p.table, phase, f);
-    if (learned) RVSP_TRY(select_learned(p.table, phase, f, s));
+    if (learned) RVSP_TRY(select_learned(d, p.table, phase, f, s));
   }
   for (const rvsp_bin_choice_t& c : d->choices) {
     if (c.phase != phase || c.bin < 1 || c.bin >= p.table.rule.nbins) continue;
@@ -394,8 +410,7 @@ rvsp_status_t plan_keep(rvsp_spgemm_descr* d, int M, hipStream_t s) {
   long long total = 0;
   RVSP_HIP_TRY(hipMemcpyAsync(&total, d->total.p, 8, hipMemcpyDeviceToHost, s));
   RVSP_HIP_TRY(hipStreamSynchronize(s));
-  size_t free_b = 0, total_b = 0;
-  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  const size_t free_b = device_free_bytes();
   const size_t bytes = (size_t)total * 4;
   const size_t budget = std::min<size_t>((size_t)16 << 30, free_b / 4);
   if (bytes > budget || ensure(d->keep, bytes) != RVSP_SUCCESS) return RVSP_SUCCESS;
@@ -624,8 +639,7 @@ rvsp_status_t plan_map(rvsp_spgemm_descr* d, hipStream_t s) {
   RVSP_HIP_TRY(hipMemcpyAsync(&total, d->total.p, 8, hipMemcpyDeviceToHost, s));
   RVSP_HIP_TRY(hipStreamSynchronize(s));
   const size_t bytes = (size_t)total * width;
-  size_t free_b = 0, total_b = 0;
-  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  const size_t free_b = device_free_bytes();
   const size_t budget = std::min<size_t>((size_t)8 << 30, free_b / 4);
   // 64 bytes of slack: the map kernels stage a segment's entries with
   // aligned 16-byte loads that may read up to 15 bytes past its end.
@@ -719,7 +733,8 @@ void rvsp_spgemm_descr_destroy(rvsp_spgemm_descr_t descr) {
                     &d->map, &d->poff, &d->map_weights, &d->keep, &d->koff,
                     &d->keep_w, &d->split_cnt, &d->split_off, &d->split_sp,
                     &d->check_key};
-  for (DevBuf* b : bufs) release(*b);
+  // The streams are drained above: the buffers go back to the cache idle.
+  for (DevBuf* b : bufs) release_idle(*b);
   destroy_graph(d);
   destroy_batch(d);
   if (d->own_stream) (void)hipStreamDestroy(d->own_stream);

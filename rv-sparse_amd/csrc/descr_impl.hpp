@@ -21,8 +21,18 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// Device buffers come from the library's cache (devpool.cpp). ensure grows
+// b to at least `bytes` (contents are not kept); release hands b back as
+// possibly still in use by queued work; release_idle when the caller knows
+// the GPU is done with it.
 rvsp_status_t ensure(DevBuf& b, size_t bytes);
 void release(DevBuf& b);
+void release_idle(DevBuf& b);
+hipError_t pool_alloc(void** out, size_t* bytes_io);
+void pool_free(void* ptr, size_t bytes, bool idle);
+size_t pool_idle_bytes();
+// Free device memory including the library's cached buffers.
+size_t device_free_bytes();
 rvsp_status_t device_ready();
 
 // One phase's bin table plus the rows assigned to each bin.
@@ -77,8 +87,11 @@ struct rvsp_spgemm_descr {
   rvsp::DevBuf gbits, gdense;
   rvsp::GlobalSlots gs_bits, gs_dense;
 
-  // Per-bin feature scratch (tuned and learned selectors).
+  // Per-bin feature scratch (tuned and learned selectors); learn_feat holds
+  // the selector network's parameters (of model version learn_version; 0 =
+  // none) and one prediction's inputs and outputs.
   rvsp::DevBuf learn_feat, bin_stats;
+  unsigned long long learn_version = 0;
 
   // Position map of the MAP numeric bins (structure cache for repeated
   // numeric-only computes): offsets, entries, coverage and validity.

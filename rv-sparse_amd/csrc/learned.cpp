@@ -222,12 +222,9 @@ struct Model {
   std::vector<float> params;
   float mu[kMlpF];
   float sigma[kMlpF];
-  void* dev_params = nullptr;  // device copy for prediction
-  // Prediction scratch for one phase's bins (kept: selection runs under the
-  // lock, and an allocation per call would synchronise the device).
-  float* dev_x = nullptr;
-  unsigned long long* dev_mask = nullptr;
-  int* dev_cls = nullptr;
+  // Bumped whenever params/mu/sigma change, so descriptors know when their
+  // device copy of the parameters is stale.
+  unsigned long long version = 1;
 };
 Model g_model;
 
@@ -241,11 +238,6 @@ std::string default_model_path() {
            "/learned_model.txt";
   }
   return "learned_model.txt";
-}
-
-void drop_device_copy_locked() {
-  if (g_model.dev_params) (void)hipFree(g_model.dev_params);
-  g_model.dev_params = nullptr;
 }
 
 int load_model_locked(const std::string& path) {
@@ -265,7 +257,7 @@ int load_model_locked(const std::string& path) {
   for (int i = 0; ok && i < kMlpP; ++i) ok = fscanf(f, "%f", &m.params[i]) == 1;
   fclose(f);
   if (!ok) return -1;
-  drop_device_copy_locked();
+  ++g_model.version;
   g_model.params = m.params;
   memcpy(g_model.mu, m.mu, sizeof m.mu);
   memcpy(g_model.sigma, m.sigma, sizeof m.sigma);
@@ -419,10 +411,25 @@ void record_sample(const rvsp_learned_sample_t& s) {
 // ---------------------------------------------------------------------------
 // Selection (work_estimation with RVSP_SPGEMM_ALGO_HIP_LEARNED).
 
-rvsp_status_t select_learned(PhaseTable& t, int phase, const std::vector<BinFeatures>& f,
-                             hipStream_t s) {
-  std::lock_guard<std::mutex> lock(g_mu);
-  if (!model_ready_locked()) return RVSP_SUCCESS;  // no model: the fixed table
+rvsp_status_t select_learned(rvsp_spgemm_descr* d, PhaseTable& t, int phase,
+                             const std::vector<BinFeatures>& f, hipStream_t s) {
+  // The model is copied out under the lock; the device work below runs on
+  // the descriptor's own scratch without it, so concurrent estimations (a
+  // batch's lanes) never wait for each other's streams.
+  float mu[kMlpF], sigma[kMlpF];
+  std::vector<float> params;
+  unsigned long long version = 0;
+  {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (!model_ready_locked()) return RVSP_SUCCESS;  // no model: the fixed table
+    memcpy(mu, g_model.mu, sizeof mu);
+    memcpy(sigma, g_model.sigma, sizeof sigma);
+    version = g_model.version;
+    if (version != d->learn_version) params = g_model.params;
+  }
+  Model norm;
+  memcpy(norm.mu, mu, sizeof mu);
+  memcpy(norm.sigma, sigma, sizeof sigma);
   std::vector<int> bins;
   std::vector<float> x;
   std::vector<unsigned long long> mask;
@@ -441,33 +448,44 @@ rvsp_status_t select_learned(PhaseTable& t, int phase, const std::vector<BinFeat
     float raw[kMlpF];
     bin_feature_vector(phase, b, t.rule.bound[b], f[b], raw);
     float nx[kMlpF];
-    normalise(raw, g_model, nx);
+    normalise(raw, norm, nx);
     bins.push_back(b);
     x.insert(x.end(), nx, nx + kMlpF);
     mask.push_back(m);
   }
   if (bins.empty()) return RVSP_SUCCESS;
-  if (!g_model.dev_params) {
-    RVSP_HIP_TRY(hipMalloc(&g_model.dev_params, (size_t)kMlpP * 4));
-    RVSP_HIP_TRY(hipMemcpy(g_model.dev_params, g_model.params.data(), (size_t)kMlpP * 4,
-                           hipMemcpyHostToDevice));
+  // Scratch layout: params | x (kMaxBins rows) | mask | cls.
+  const size_t off_x = (size_t)kMlpP * 4;
+  const size_t off_mask = off_x + (size_t)kMaxBins * kMlpF * 4;
+  const size_t off_cls = off_mask + (size_t)kMaxBins * 8;
+  const size_t bytes = off_cls + (size_t)kMaxBins * 4;
+  if (d->learn_feat.bytes < bytes) d->learn_version = 0;  // reallocated: params lost
+  RVSP_TRY(ensure(d->learn_feat, bytes));
+  char* base = (char*)d->learn_feat.p;
+  if (version != d->learn_version) {
+    if (params.empty()) {
+      std::lock_guard<std::mutex> lock(g_mu);
+      if (!g_model.ready) return RVSP_SUCCESS;
+      params = g_model.params;
+      version = g_model.version;
+    }
+    RVSP_HIP_TRY(hipMemcpyAsync(base, params.data(), (size_t)kMlpP * 4,
+                                hipMemcpyHostToDevice, s));
   }
   const int n = (int)bins.size();  // <= kMaxBins
-  if (!g_model.dev_x) {
-    RVSP_HIP_TRY(hipMalloc(&g_model.dev_x, (size_t)kMaxBins * kMlpF * sizeof(float)));
-    RVSP_HIP_TRY(hipMalloc(&g_model.dev_mask, (size_t)kMaxBins * sizeof(unsigned long long)));
-    RVSP_HIP_TRY(hipMalloc(&g_model.dev_cls, (size_t)kMaxBins * sizeof(int)));
-  }
-  RVSP_HIP_TRY(hipMemcpyAsync(g_model.dev_x, x.data(), x.size() * sizeof(float),
+  RVSP_HIP_TRY(hipMemcpyAsync(base + off_x, x.data(), x.size() * sizeof(float),
                               hipMemcpyHostToDevice, s));
-  RVSP_HIP_TRY(hipMemcpyAsync(g_model.dev_mask, mask.data(), mask.size() * sizeof(unsigned long long),
+  RVSP_HIP_TRY(hipMemcpyAsync(base + off_mask, mask.data(),
+                              mask.size() * sizeof(unsigned long long),
                               hipMemcpyHostToDevice, s));
-  RVSP_HIP_TRY(launch_mlp_forward((const float*)g_model.dev_params, g_model.dev_x,
-                                  g_model.dev_mask, n, nullptr, g_model.dev_cls, s));
+  RVSP_HIP_TRY(launch_mlp_forward((const float*)base, (const float*)(base + off_x),
+                                  (const unsigned long long*)(base + off_mask), n, nullptr,
+                                  (int*)(base + off_cls), s));
   std::vector<int> cls(n);
-  RVSP_HIP_TRY(hipMemcpyAsync(cls.data(), g_model.dev_cls, (size_t)n * 4, hipMemcpyDeviceToHost,
+  RVSP_HIP_TRY(hipMemcpyAsync(cls.data(), base + off_cls, (size_t)n * 4, hipMemcpyDeviceToHost,
                               s));
   RVSP_HIP_TRY(hipStreamSynchronize(s));
+  d->learn_version = version;
   for (int i = 0; i < n; ++i) {
     if (cls[i] < 0) continue;
     const int b = bins[i];
@@ -550,7 +568,7 @@ rvsp_status_t rvsp_learned_fit(int32_t first, int32_t count, int32_t epochs, flo
                         lr, loss.data()));
   if (final_loss) *final_loss = loss.back();
   std::lock_guard<std::mutex> lock(g_mu);
-  drop_device_copy_locked();
+  ++g_model.version;
   g_model.params = m.params;
   memcpy(g_model.mu, m.mu, sizeof m.mu);
   memcpy(g_model.sigma, m.sigma, sizeof m.sigma);
@@ -582,7 +600,7 @@ int32_t rvsp_learned_model_load(const char* path) {
 
 void rvsp_learned_model_clear(void) {
   std::lock_guard<std::mutex> lock(g_mu);
-  drop_device_copy_locked();
+  ++g_model.version;
   g_model.ready = false;
   g_model.tried_default = true;
 }

@@ -26,8 +26,10 @@ int len_class(const BinFeatures& f);
 void select_tuned(PhaseTable& t, int phase, const std::vector<BinFeatures>& f);
 
 // HIP_LEARNED: the MLP selector's choice for every non-empty bin.
-rvsp_status_t select_learned(PhaseTable& t, int phase, const std::vector<BinFeatures>& f,
-                             hipStream_t s);
+// The model is read under the library's lock; the prediction runs on the
+// descriptor's scratch (learn_feat) with no lock held.
+rvsp_status_t select_learned(rvsp_spgemm_descr* d, PhaseTable& t, int phase,
+                             const std::vector<BinFeatures>& f, hipStream_t s);
 
 // Output classes of the selector network (rv_sparse_learned.h). A class is a
 // kernel shape relative to the fixed table's bin: its kind, lanes per row and

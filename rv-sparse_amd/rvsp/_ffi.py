@@ -139,6 +139,8 @@ PROTOTYPES = {
     "rvsp_hip_free": (C.c_int, [P]),
     "rvsp_hip_memcpy": (C.c_int, [P, P, SZ, C.c_int]),
     "rvsp_hip_synchronize": (C.c_int, []),
+    "rvsp_hip_pool_release": (C.c_int, []),
+    "rvsp_hip_pool_cached_bytes": (SZ, []),
     "rvsp_hip_device_count": (I32, []),
     "rvsp_csr_check_device": (C.c_int, [I32, I32, I32, P, P, I32P, I32P]),
     # include/rv_sparse_learned.h

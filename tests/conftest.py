@@ -3,6 +3,10 @@ import sys
 
 import pytest
 
+# Every device buffer the library hands out is filled with 0xFF bytes, so no
+# kernel can pass by relying on memory the driver happened to clear.
+os.environ.setdefault("RVSP_POOL_POISON", "1")
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (os.path.join(ROOT, "rv-sparse_amd"), ROOT, os.path.dirname(os.path.abspath(__file__))):
     if p not in sys.path:
