@@ -481,6 +481,8 @@ def bench_powerlaw(hip, args, F):
     plans = {"plain_fixed": run_plain,
              "batched_fixed": lambda: run_batched(F.ALGO_HIP_FIXED),
              "learned_batched": lambda: run_batched(F.ALGO_HIP_LEARNED)}
+    if args.powerlaw_plans:  # profiling runs: a subset of the plans
+        plans = {k: v for k, v in plans.items() if k in args.powerlaw_plans.split(",")}
     for name, fn in plans.items():  # untimed: first use of every kernel, model, stream
         sentinel()
         fn()
@@ -543,7 +545,8 @@ def bench_powerlaw(hip, args, F):
     flops = 2.0 * products
     med = {k: spread([e + c for e, c, _ in v]) for k, v in runs.items()}
     ms = {k: round(m * 1e3, 1) for k, (m, _) in med.items()}
-    lb = med["learned_batched"][0]
+    ratio = (lambda a, b: round(med[a][0] / med[b][0], 3) if a in med and b in med else None)
+    lb = med["learned_batched"][0] if "learned_batched" in med else min(m for m, _ in med.values())
     # Algorithmic bytes of the whole product (A = B counted once; C's row
     # pointers once per row plus one per block).
     alg_bytes = 4 * (A.rows + 1) + 8 * A.nnz + 4 * (A.rows + nb) + 8 * total
@@ -558,10 +561,10 @@ def bench_powerlaw(hip, args, F):
                           "compute": round(statistics.median(c for _, c, _ in v) * 1e3, 1)}
                       for k, v in runs.items()},
         "all_runs_ms": {k: [round((e + c) * 1e3, 1) for e, c, _ in v] for k, v in runs.items()},
-        "groups": runs["learned_batched"][-1][2],
+        "groups": {k: v[-1][2] for k, v in runs.items()},
         "gflops": {k: round(flops / m / 1e9, 2) for k, (m, _) in med.items()},
-        "speedup_learned_batched_vs_plain": round(med["plain_fixed"][0] / lb, 3),
-        "speedup_learned_vs_fixed_batched": round(med["batched_fixed"][0] / lb, 3),
+        "speedup_learned_batched_vs_plain": ratio("plain_fixed", "learned_batched"),
+        "speedup_learned_vs_fixed_batched": ratio("batched_fixed", "learned_batched"),
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
                      "algorithmic_bytes": alg_bytes, "scope": "whole one-shot product "
@@ -815,6 +818,9 @@ def main():
     ap.add_argument("--batch-algo", choices=["fixed", "learned"], default="learned")
     ap.add_argument("--rmat-scale", type=int, default=20)
     ap.add_argument("--powerlaw-runs", type=int, default=3)
+    ap.add_argument("--powerlaw-plans", default=None,
+                    help="comma list of plain_fixed, batched_fixed, learned_batched "
+                         "(profiling runs; default all)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run device-to-device copies of this many MiB (a known "
                          "byte count for scaling the HBM traffic counters), then exit")
