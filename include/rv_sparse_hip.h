@@ -227,6 +227,19 @@ rvsp_status_t rvsp_csr_check_device(int32_t rows, int32_t cols, int32_t nnz,
                                     rvsp_csr_status_t *status_out,
                                     int32_t *bad_row_out);
 
+/* rvsp_csr_check_device without any device allocation. No counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (tight serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * verdicts and return values as rvsp_csr_check_device. Its scratch (the
+ * check's 8-byte key) is a 256-byte static device array per device, held
+ * until the process ends; no matrix size exceeds it. Calls that overlap on
+ * one device share that array and race. */
+rvsp_status_t rvsp_csr_check_device_noalloc(int32_t rows, int32_t cols, int32_t nnz,
+                                            const int32_t *row_ptr,
+                                            const int32_t *col_idx,
+                                            rvsp_csr_status_t *status_out,
+                                            int32_t *bad_row_out);
+
 /* Number of usable gfx950 devices (0 when none; never an error for "no
  * GPU"). */
 int32_t rvsp_hip_device_count(void);

@@ -122,6 +122,14 @@ RowStats row_stats(rvsp_spgemm_descr* d) {
 rvsp_status_t csr_check_device(int rows, int cols, int nnz, const int* rp, const int* ci,
                                DevBuf& key_buf, hipStream_t s, rvsp_csr_status_t* verdict,
                                int* bad_row) {
+  if (rows > 0 && rp != NULL) RVSP_TRY(ensure(key_buf, 8));
+  return csr_check_device_key(rows, cols, nnz, rp, ci, (unsigned long long*)key_buf.p, s,
+                              verdict, bad_row);
+}
+
+rvsp_status_t csr_check_device_key(int rows, int cols, int nnz, const int* rp, const int* ci,
+                                   unsigned long long* dkey, hipStream_t s,
+                                   rvsp_csr_status_t* verdict, int* bad_row) {
   *verdict = RVSP_CSR_OK;
   *bad_row = -1;
   if (rows < 0 || cols < 0 || nnz < 0 || rp == NULL) {
@@ -136,8 +144,6 @@ rvsp_status_t csr_check_device(int rows, int cols, int nnz, const int* rp, const
     return RVSP_SUCCESS;
   }
   if (rows == 0) return RVSP_SUCCESS;
-  RVSP_TRY(ensure(key_buf, 8));
-  unsigned long long* dkey = (unsigned long long*)key_buf.p;
   unsigned long long key = ~0ull;
   RVSP_HIP_TRY(hipMemcpyAsync(dkey, &key, 8, hipMemcpyHostToDevice, s));
   RVSP_HIP_TRY(launch_csr_check_rows(rows, nnz, rp, ci != NULL, dkey, s));

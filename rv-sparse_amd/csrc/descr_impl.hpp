@@ -167,6 +167,11 @@ rvsp_status_t promote_to_map(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr
 rvsp_status_t csr_check_device(int rows, int cols, int nnz, const int* rp, const int* ci,
                                DevBuf& key_buf, hipStream_t s, rvsp_csr_status_t* verdict,
                                int* bad_row);
+// The same check with the caller's 8-byte device key (a 64-bit atomic
+// minimum, set before each pass); allocates nothing.
+rvsp_status_t csr_check_device_key(int rows, int cols, int nnz, const int* rp, const int* ci,
+                                   unsigned long long* dkey, hipStream_t s,
+                                   rvsp_csr_status_t* verdict, int* bad_row);
 // rvsp_spgemm_csr for FP64 x FP64 -> FP64 (i8 false) and INT8 x INT8 ->
 // INT32 (i8 true) operands (legacy.cpp).
 rvsp_status_t oneshot_legacy(const rvsp_csr_matrix_t* A, const rvsp_csr_matrix_t* B,

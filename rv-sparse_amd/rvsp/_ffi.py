@@ -143,6 +143,7 @@ PROTOTYPES = {
     "rvsp_hip_pool_cached_bytes": (SZ, []),
     "rvsp_hip_device_count": (I32, []),
     "rvsp_csr_check_device": (C.c_int, [I32, I32, I32, P, P, I32P, I32P]),
+    "rvsp_csr_check_device_noalloc": (C.c_int, [I32, I32, I32, P, P, I32P, I32P]),
     # include/rv_sparse_learned.h
     "rvsp_learned_class_info": (C.c_int, [I32, C.POINTER(LearnedClass)]),
     "rvsp_learned_sample_count": (I32, []),
