@@ -182,10 +182,129 @@ def word_checksum(t, offset=0, chunk=1 << 27):
     return total % (1 << 64)
 
 
+def bench_uniform(hip, args, F):
+    """C = A * A for A 4096 x 4096 with 16 uniform distinct columns per row,
+    U[-1, 1) (the plain CPU-runnable workload): one product is a few
+    microseconds of GPU work, so it is measured as an aggregate of
+    `--uniform-pairs` independent products (seeds 0, 1, ...) in one batched
+    descriptor whose numeric-only computes replay a captured HIP graph, next
+    to one product's one-shot time and its numeric-only reuse through a
+    single descriptor."""
+    import rvsp
+    from oracle import oracle
+    from rvsp import gen
+    from rvsp.batch import Batch, BatchOutput
+    n_pairs = args.uniform_pairs
+    mats = [gen.uniform_k(4096, 16, seed=s) for s in range(n_pairs)]
+    A = mats[0]
+    dmats = [rvsp.DeviceCsr.upload(M) for M in mats]
+    dA = dmats[0]
+    # One product: one-shot (estimation + first compute) and numeric-only.
+    oneshot = []
+    for _ in range(args.uniform_runs + 1):
+        d = rvsp.Descriptor(algo=F.ALGO_HIP_LEARNED, memory=F.MEMORY_DEVICE)
+        hip.sync()
+        t0 = time.perf_counter()
+        nnz = d.estimate(dA, dA)
+        dC = rvsp.DeviceCsr(A.rows, A.cols, nnz)
+        cm = dC.struct(nnz)
+        F.check(d.compute(dA, dA, cm), "compute")
+        oneshot.append(time.perf_counter() - t0)
+        if _ == 0:
+            # Numeric-only reuse on this descriptor (asynchronous launches).
+            F.check(d.compute(dA, dA, cm), "compute")
+            d.set_async(True)
+            reps = 200
+            hip.sync()
+            t0 = time.perf_counter()
+            for _r in range(reps):
+                F.check(d.compute(dA, dA, cm), "compute")
+            hip.sync()
+            t_num_single = (time.perf_counter() - t0) / reps
+            got = dC.download()
+        d.close()
+        dC.free()
+    t_oneshot = statistics.median(oneshot[1:])
+    rp, ci, v = oracle.spgemm(A, A)
+    ok = (got.row_ptr.tobytes() == rp.tobytes() and got.col_idx.tobytes() == ci.tobytes()
+          and got.values.tobytes() == v.tobytes())
+    err = oracle.f64_error(A, A, rp, ci, got.values)
+    products1 = int(np.diff(A.row_ptr)[A.col_idx].sum())
+    # Aggregate: every product of the batch, numeric-only, replayed graph.
+    bt = Batch(algo=F.ALGO_HIP_LEARNED)
+    nnz_b = bt.estimate(dmats, dmats)
+    out = BatchOutput([M.rows for M in mats], [M.cols for M in mats], nnz_b)
+    for _ in range(3):  # first compute, position maps, capture
+        F.check(bt.compute(dmats, dmats, out.structs), "batch compute")
+    bt.set_async(True)
+    reps = args.uniform_graph_reps
+    s = bt.stream()
+    e0, e1 = hip.event(), hip.event()
+    hip.sync()
+    t0 = time.perf_counter()
+    hip.record(e0, s)
+    for _ in range(reps):
+        F.check(bt.compute(dmats, dmats, out.structs), "batch compute")
+    hip.record(e1, s)
+    hip.sync()
+    t_graph = (time.perf_counter() - t0) / reps
+    t_graph_ev = hip.elapsed_ms(e0, e1) * 1e-3 / reps
+    graphs = bt.graph_launches()
+    last = out.download(n_pairs - 1)
+    rp2, ci2, v2 = oracle.spgemm(mats[-1], mats[-1])
+    ok &= (last.row_ptr.tobytes() == rp2.tobytes() and last.col_idx.tobytes() == ci2.tobytes()
+           and last.values.tobytes() == v2.tobytes())
+    out.free()
+    bt.close()
+    for M in dmats:
+        M.free()
+    products = sum(int(np.diff(M.row_ptr)[M.col_idx].sum()) for M in mats)
+    nnz_total = int(sum(nnz_b))
+    alg_bytes = sum(4 * (M.rows + 1) + 8 * M.nnz + 4 * (M.rows + 1) for M in mats) + 8 * nnz_total
+    ach = alg_bytes / t_graph_ev / 1e9
+    cpu = None
+    if not args.no_cpu and oracle.ref_available():
+        t_ref = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            oracle.ref_spgemm(A, A)
+            t_ref.append(time.perf_counter() - t0)
+        from rvsp import _ffi as FF
+        times, _ = oracle.time_reference_numeric(A, A, 20, FF.CsrMatrix)
+        cpu = {"value": round(2 * products1 / statistics.median(times) / 1e9, 4),
+               "unit": "GFLOP/s", "cores": 1, "kind": "reference",
+               "sample": "numeric-only multiply of the seed-0 product through the reference "
+                         "descriptor (fill once, untimed), median of 20, one thread",
+               "oneshot_ms": round(statistics.median(t_ref) * 1e3, 3),
+               "numeric_ms": round(statistics.median(times) * 1e3, 4)}
+    return {
+        "workload": f"uniform16_n4096_AxA_x{n_pairs}_graph",
+        "pairs": n_pairs, "products_per_pair": products1, "products": products,
+        "nnz_c": nnz_total,
+        "single_product": {"oneshot_ms": round(t_oneshot * 1e3, 3),
+                           "numeric_ms": round(t_num_single * 1e3, 4),
+                           "numeric_gflops": round(2 * products1 / t_num_single / 1e9, 2)},
+        "graph_numeric_ms": round(t_graph * 1e3, 4),
+        "graph_numeric_gflops": round(2 * products / t_graph / 1e9, 2),
+        "graph_launches": graphs,
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
+                     "algorithmic_bytes": alg_bytes, "kernel_ms": round(t_graph_ev * 1e3, 4),
+                     "scope": "one graph replay (numeric-only multiply of every pair), "
+                              "HIP events on the batch's stream"},
+        "fp64_error": oracle.merge_f64_errors([err]),
+        "verified_bit_exact": bool(ok),
+        "cpu_baseline": cpu,
+    }
+
+
 def bench_batched(hip, args, F):
     """2048 independent pairs of uneven shapes: the plain path (one
     descriptor per pair, FIXED) against one batched descriptor (one set of
-    launches per phase; numeric-only repeats replayed from a HIP graph)."""
+    launches per phase; numeric-only repeats replayed from a HIP graph).
+    Both sides get the same untimed warm-up before their timed numeric-only
+    repeats (the first numeric-only compute, which builds the position maps,
+    and one more); every run writes outputs pre-filled with a sentinel."""
     import torch
     import rvsp
     from oracle import oracle
@@ -218,8 +337,20 @@ def bench_batched(hip, args, F):
             off += int(nnz[p])
         return ci, val, rps, structs
 
+    def sentinel(ci, val, rps):
+        ci.fill_(-1)
+        val.view(torch.int32).fill_(-1)
+        for r in rps:
+            r.fill_(-1)
+        torch.cuda.synchronize()
+
+    def sums(ci, val, rps):
+        return (word_checksum(ci), word_checksum(val),
+                sum(word_checksum(r) for r in rps) % (1 << 64))
+
     a_st = [A.struct() for A, _ in pairs]
     b_st = [B.struct() for _, B in pairs]
+    reps = args.batch_repeats
     # Plain: a descriptor per pair, each call synchronous.
     descs = []
     hip.sync()
@@ -233,91 +364,117 @@ def bench_batched(hip, args, F):
         nnz_plain.append(nz.value)
         descs.append(d)
     t_plain_est = time.perf_counter() - t0
-    ci_p, val_p, rps_p, out_p = alloc_out(nnz_plain)
+    ci_o, val_o, rps_o, out = alloc_out(nnz_plain)
     tok = C.c_void_p(descs[0].ws.ctypes.data)
+
+    def plain_all():
+        for p in range(n):
+            F.check(L.rvsp_spgemm_compute(descs[p].handle, C.byref(a_st[p]), C.byref(b_st[p]),
+                                          C.byref(out[p]), tok), "compute")
+
+    sentinel(ci_o, val_o, rps_o)
     hip.sync()
     t0 = time.perf_counter()
-    for p in range(n):
-        F.check(L.rvsp_spgemm_compute(descs[p].handle, C.byref(a_st[p]), C.byref(b_st[p]),
-                                      C.byref(out_p[p]), tok), "compute")
+    plain_all()
     t_plain_first = time.perf_counter() - t0
-    reps = args.batch_repeats
+    sums_first = sums(ci_o, val_o, rps_o)
+    plain_all()  # untimed: first numeric-only compute (position maps built)
+    plain_all()  # untimed: second, as the batched side gets
+    # Numeric-only computes rewrite the values only (the structure stays from
+    # the first compute, checked above): they must rewrite all of them.
+    val_o.view(torch.int32).fill_(-1)
+    torch.cuda.synchronize()
     hip.sync()
     t0 = time.perf_counter()
     for _ in range(reps):
-        for p in range(n):
-            F.check(L.rvsp_spgemm_compute(descs[p].handle, C.byref(a_st[p]), C.byref(b_st[p]),
-                                          C.byref(out_p[p]), tok), "compute")
+        plain_all()
     t_plain_num = (time.perf_counter() - t0) / reps
+    sums_plain = sums(ci_o, val_o, rps_o)
     for d in descs:
         d.close()
     print(f"# batched: plain estimate {t_plain_est * 1e3:.1f} ms, first compute "
           f"{t_plain_first * 1e3:.1f} ms, numeric {t_plain_num * 1e3:.1f} ms",
           file=sys.stderr, flush=True)
-    sums_plain = (word_checksum(ci_p), word_checksum(val_p),
-                  sum(word_checksum(r) for r in rps_p) % (1 << 64))
     # Batched.
     bt = Batch(algo=F.ALGO_HIP_LEARNED if args.batch_algo == "learned" else F.ALGO_HIP_FIXED)
+    As, Bs = [A for A, _ in pairs], [B for _, B in pairs]
     hip.sync()
     t0 = time.perf_counter()
-    As, Bs = [A for A, _ in pairs], [B for _, B in pairs]
     nnz_b = bt.estimate(As, Bs)
     t_b_est = time.perf_counter() - t0
-    del ci_p, val_p, rps_p
-    ci_b, val_b, rps_b, out_b = alloc_out(nnz_b)
+    sentinel(ci_o, val_o, rps_o)
     hip.sync()
     t0 = time.perf_counter()
-    F.check(bt.compute(As, Bs, out_b), "batch compute")
+    F.check(bt.compute(As, Bs, out), "batch compute")
     t_b_first = time.perf_counter() - t0
-    F.check(bt.compute(As, Bs, out_b), "batch compute")  # numeric-only: position maps built
-    F.check(bt.compute(As, Bs, out_b), "batch compute")  # captured here
+    same_first = list(nnz_b) == nnz_plain and sums(ci_o, val_o, rps_o) == sums_first
+    F.check(bt.compute(As, Bs, out), "batch compute")  # untimed: position maps built
+    F.check(bt.compute(As, Bs, out), "batch compute")  # untimed: captured here
+    val_o.view(torch.int32).fill_(-1)  # the replays rewrite every value
+    torch.cuda.synchronize()
     hip.sync()
     t0 = time.perf_counter()
     for _ in range(reps):
-        F.check(bt.compute(As, Bs, out_b), "batch compute")
+        F.check(bt.compute(As, Bs, out), "batch compute")
     t_b_num = (time.perf_counter() - t0) / reps
     groups, graphs = bt.groups(), bt.graph_launches()
     bt.close()
-    sums_b = (word_checksum(ci_b), word_checksum(val_b),
-              sum(word_checksum(r) for r in rps_b) % (1 << 64))
-    same = list(nnz_b) == nnz_plain and sums_b == sums_plain
-    # Oracle on a few pairs.
+    same = same_first and sums(ci_o, val_o, rps_o) == sums_plain
+    # Oracle (bits) and FP64 error on a few pairs.
     oracle_ok = True
+    errs = []
     for p in range(0, n, max(1, n // 4)):
         A, B = pairs[p][0].to_host(), pairs[p][1].to_host()
         rp, ci, v = oracle.spgemm(A, B)
-        o = out_b[p]
+        o = out[p]
         k = int(nnz_b[p])
-        got_rp = rps_b[p].cpu().numpy()
-        got_ci = torch.tensor([], dtype=torch.int32) if k == 0 else \
-            ci_b[(o.col_idx - ci_b.data_ptr()) // 4:(o.col_idx - ci_b.data_ptr()) // 4 + k].cpu()
-        got_v = val_b[(o.values - val_b.data_ptr()) // 4:(o.values - val_b.data_ptr()) // 4 + k].cpu()
-        oracle_ok &= (got_rp.tobytes() == rp.tobytes() and got_ci.numpy().tobytes() == ci.tobytes()
-                      and got_v.numpy().tobytes() == v.tobytes())
+        base = (o.col_idx - ci_o.data_ptr()) // 4
+        got_rp = rps_o[p].cpu().numpy()
+        got_ci = ci_o[base:base + k].cpu().numpy()
+        got_v = val_o[base:base + k].cpu().numpy()
+        oracle_ok &= (got_rp.tobytes() == rp.tobytes() and got_ci.tobytes() == ci.tobytes()
+                      and got_v.tobytes() == v.tobytes())
+        errs.append(oracle.f64_error(A, B, rp, ci, got_v))
     cpu = None
     if not args.no_cpu:
         host_pairs = ((A.to_host(), B.to_host()) for A, B in pairs)
         spent, prods, npairs, kind = cpu_oneshot(host_pairs, args.cpu_seconds / 2)
         cpu = {"value": round(2 * prods / spent / 1e9, 4), "unit": "GFLOP/s", "cores": 1,
                "kind": kind, "sample": f"one-shot products of the first {npairs} pairs"}
-    del ci_b, val_b, rps_b, keep
+    nnz_total = int(sum(nnz_b))
+    del ci_o, val_o, rps_o, keep
     torch.cuda.empty_cache()
     flops = 2.0 * products
     oneshot_plain = t_plain_est + t_plain_first
     oneshot_b = t_b_est + t_b_first
+    # Algorithmic bytes of one numeric-only multiply of every pair (A and B
+    # read, C's structure read, C's values written): A != B in every pair.
+    alg_bytes = sum(4 * (A.rows + 1) + 8 * A.nnz + 4 * (B.rows + 1) + 8 * B.nnz
+                    + 4 * (A.rows + 1) for A, B in pairs) + 8 * nnz_total
+    ach = alg_bytes / t_b_num / 1e9
     return {
         "workload": f"lognormal_pairs{n}_n256-65536",
-        "pairs": n, "products": products, "nnz_c": int(sum(nnz_b)), "groups": groups,
+        "pairs": n, "products": products, "nnz_c": nnz_total, "groups": groups,
         "algo_batched": args.batch_algo,
         "oneshot_ms": {"plain": round(oneshot_plain * 1e3, 2), "batched": round(oneshot_b * 1e3, 2)},
+        "oneshot_phases_ms": {"plain": {"estimate": round(t_plain_est * 1e3, 2),
+                                        "first_compute": round(t_plain_first * 1e3, 2)},
+                              "batched": {"estimate": round(t_b_est * 1e3, 2),
+                                          "first_compute": round(t_b_first * 1e3, 2)}},
         "oneshot_gflops": {"plain": round(flops / oneshot_plain / 1e9, 2),
                            "batched": round(flops / oneshot_b / 1e9, 2)},
         "speedup_oneshot": round(oneshot_plain / oneshot_b, 3),
         "numeric_ms": {"plain": round(t_plain_num * 1e3, 3), "batched_graph": round(t_b_num * 1e3, 3)},
+        "plain_ms_per_call": round(t_plain_num * 1e3 / n, 4),
         "numeric_gflops": {"plain": round(flops / t_plain_num / 1e9, 2),
                            "batched_graph": round(flops / t_b_num / 1e9, 2)},
         "speedup_numeric": round(t_plain_num / t_b_num, 3),
         "graph_launches": graphs,
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
+                     "algorithmic_bytes": alg_bytes,
+                     "scope": "numeric-only multiply of all pairs (graph replay)"},
+        "fp64_error": oracle.merge_f64_errors(errs),
         "verified_bit_exact": bool(same and oracle_ok),
         "cpu_baseline": cpu,
     }
@@ -337,34 +494,6 @@ def row_blocks(A, max_products):
         acc += int(o)
     cuts.append(A.rows)
     return cuts, ops
-
-
-def fp64_error(A, B, rp, ci, got, r0=0):
-    """Error of GPU values `got` (structure rp, ci of rows [r0, ...)) against
-    the FP64 oracle: the largest |c - c64| / |c64| over entries whose
-    |c64| is at least 1e-6 of their sum of |a*b| (near-total cancellation
-    leaves no relative scale; such entries are counted, not hidden), and the
-    condition-aware max |c - c64| / sum_k |a_ik b_kj| over all entries."""
-    from oracle import oracle
-    v64, mag = oracle.spgemm_f64(A, B, np.asarray(rp, dtype=np.int32), ci, r0)
-    d = np.abs(got.astype(np.float64) - v64)
-    scaled = np.abs(v64) >= 1e-6 * mag
-    rel = float((d[scaled] / np.abs(v64[scaled])).max()) if scaled.any() else 0.0
-    cond = float((d / np.where(mag > 0, mag, 1.0)).max()) if d.size else 0.0
-    return {"entries": int(d.size), "max_rel_err": rel, "max_cond_err": cond,
-            "cancelled_entries": int((~scaled).sum())}
-
-
-def merge_err(errs):
-    """Combines fp64_error results of several samples."""
-    errs = [e for e in errs if e["entries"]]
-    if not errs:
-        return None
-    return {"entries": sum(e["entries"] for e in errs),
-            "max_rel_err": max(e["max_rel_err"] for e in errs),
-            "max_cond_err": max(e["max_cond_err"] for e in errs),
-            "cancelled_entries": sum(e["cancelled_entries"] for e in errs),
-            "tolerance": 1e-5}
 
 
 def spread(xs):
@@ -517,7 +646,7 @@ def bench_powerlaw(hip, args, F):
         got_v = val_out[off_blk[b] + s:off_blk[b] + e].cpu().numpy()
         oracle_ok &= (np.array_equal(crp[lo:hi + 1] - s, want_rp) and
                       got_ci.tobytes() == want_ci.tobytes() and got_v.tobytes() == want_v.tobytes())
-        errs.append(fp64_error(A, A, want_rp, want_ci, got_v, int(r0)))
+        errs.append(oracle.f64_error(A, A, want_rp, want_ci, got_v, int(r0)))
     cpu = None
     if not args.no_cpu:
         # Reference one-shot on whole row blocks of 256 rows spread over A.
@@ -569,7 +698,7 @@ def bench_powerlaw(hip, args, F):
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
                      "algorithmic_bytes": alg_bytes, "scope": "whole one-shot product "
                      "(learned_batched, estimation + compute)"},
-        "fp64_error": merge_err(errs),
+        "fp64_error": oracle.merge_f64_errors(errs),
         "verified_bit_exact": bool(same and oracle_ok),
         "cpu_baseline": cpu,
     }
@@ -587,45 +716,89 @@ def bench_banded(hip, args, F):
     products = int(np.diff(B.row_ptr)[A.col_idx].sum())
     print(f"# banded: A {A.rows}x{A.cols} nnz {A.nnz}, B nnz {B.nnz}, products {products}",
           file=sys.stderr, flush=True)
+    import torch
+    from rvsp.api import Csr
     dA, dB = rvsp.DeviceCsr.upload(A), rvsp.DeviceCsr.upload(B)
     res = {}
     outs = {}
+    c_rp = torch.empty(A.rows + 1, dtype=torch.int32, device="cuda")
+    c_ci = c_val = None
     for name, algo in (("fixed", F.ALGO_HIP_FIXED), ("learned", F.ALGO_HIP_LEARNED)):
-        d = rvsp.Descriptor(algo=algo, memory=F.MEMORY_DEVICE)
-        hip.sync()
-        t0 = time.perf_counter()
-        nnz = d.estimate(dA, dB)
-        dC = rvsp.DeviceCsr(A.rows, B.cols, nnz)
-        cm = dC.struct(nnz)
-        t_alloc = time.perf_counter()
-        F.check(d.compute(dA, dB, cm), "compute")
-        t1 = time.perf_counter()
-        oneshot = (t_alloc - t0) + (t1 - t_alloc)
-        hip.sync()
-        t0 = time.perf_counter()
-        for _ in range(5):
+        oneshot, phases = [], []
+        for run in range(args.banded_runs + 1):  # run 0 untimed: first use
+            d = rvsp.Descriptor(algo=algo, memory=F.MEMORY_DEVICE)
+            hip.sync()
+            t0 = time.perf_counter()
+            nnz = d.estimate(dA, dB)
+            t1 = time.perf_counter()
+            if c_ci is None:
+                c_ci = torch.empty(max(nnz, 1), dtype=torch.int32, device="cuda")
+                c_val = torch.empty(max(nnz, 1), dtype=torch.float32, device="cuda")
+            for t in (c_rp, c_ci, c_val.view(torch.int32)):
+                t.fill_(-1)
+            torch.cuda.synchronize()
+            cm = F.CsrMatrix(A.rows, B.cols, nnz, c_rp.data_ptr(), c_ci.data_ptr(),
+                             c_val.data_ptr(), F.DTYPE_FP32, F.FORMAT_CSR, 0)
+            hip.sync()
+            t2 = time.perf_counter()
             F.check(d.compute(dA, dB, cm), "compute")
-        num = (time.perf_counter() - t0) / 5
+            t3 = time.perf_counter()
+            if run:
+                oneshot.append((t1 - t0) + (t3 - t2))
+                phases.append((t1 - t0, t3 - t2))
+            if run < args.banded_runs:
+                d.close()
+        # Numeric-only repeats on the last descriptor: one untimed (position
+        # maps), then values rewritten from the sentinel by the timed ones.
+        F.check(d.compute(dA, dB, cm), "compute")
+        c_val.view(torch.int32).fill_(-1)
+        torch.cuda.synchronize()
+        hip.sync()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            F.check(d.compute(dA, dB, cm), "compute")
+        num = (time.perf_counter() - t0) / 20
         d.close()
-        res[name] = {"oneshot_ms": oneshot * 1e3, "numeric_ms": num * 1e3}
-        outs[name] = dC.download()
-        dC.free()
+        res[name] = {"oneshot_ms": statistics.median(oneshot) * 1e3, "numeric_ms": num * 1e3,
+                     "oneshot_spread": spread(oneshot)[1],
+                     "estimate_ms": statistics.median(e for e, _ in phases) * 1e3,
+                     "first_compute_ms": statistics.median(c for _, c in phases) * 1e3}
+        outs[name] = Csr(A.rows, B.cols, c_rp.cpu().numpy(), c_ci[:nnz].cpu().numpy(),
+                         c_val[:nnz].cpu().numpy())
     same = all(getattr(outs["fixed"], k).tobytes() == getattr(outs["learned"], k).tobytes()
                for k in ("row_ptr", "col_idx", "values"))
     rp, ci, v = oracle.spgemm(A, B, 0, 5000)
     got = outs["fixed"]
     ok = (np.array_equal(got.row_ptr[:5001], rp) and got.col_idx[:rp[-1]].tobytes() == ci.tobytes()
           and got.values[:rp[-1]].tobytes() == v.tobytes())
+    err = oracle.f64_error(A, B, rp, ci, got.values[:rp[-1]])
     dA.free()
     dB.free()
+    del c_rp, c_ci, c_val
     flops = 2.0 * products
+    nnz_c = int(outs["fixed"].nnz)
+    alg_bytes = (4 * (A.rows + 1) + 8 * A.nnz + 4 * (B.rows + 1) + 8 * B.nnz
+                 + 4 * (A.rows + 1) + 8 * nnz_c)
+    best = min(r["numeric_ms"] for r in res.values()) * 1e-3
+    ach = alg_bytes / best / 1e9
     return {
         "workload": "banded500k_x_uniform200k_mixed_dense_rows",
         "products": products, "nnz_c": int(outs["fixed"].nnz),
         "oneshot_ms": {k: round(r["oneshot_ms"], 2) for k, r in res.items()},
+        "oneshot_phases_ms": {k: {"estimate": round(r["estimate_ms"], 2),
+                                  "first_compute": round(r["first_compute_ms"], 2)}
+                              for k, r in res.items()},
+        "oneshot_spread": {k: round(r["oneshot_spread"], 3) for k, r in res.items()},
+        "runs": args.banded_runs,
         "numeric_ms": {k: round(r["numeric_ms"], 3) for k, r in res.items()},
         "numeric_gflops": {k: round(flops / (r["numeric_ms"] * 1e-3) / 1e9, 2) for k, r in res.items()},
         "speedup_learned_vs_fixed_oneshot": round(res["fixed"]["oneshot_ms"] / res["learned"]["oneshot_ms"], 3),
+        "speedup_learned_vs_fixed_numeric": round(res["fixed"]["numeric_ms"] / res["learned"]["numeric_ms"], 3),
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
+                     "algorithmic_bytes": alg_bytes,
+                     "scope": "numeric-only multiply (faster plan)"},
+        "fp64_error": oracle.merge_f64_errors([err]),
         "verified_bit_exact": bool(same and ok),
     }
 
@@ -702,6 +875,7 @@ def bench_headline(hip, args, F):
     head_name = "tuned" if "tuned" in runs else "fixed"
 
     verified = None
+    fp64 = None
     if not args.no_verify:
         from oracle import oracle
         last = A.with_values(host_vals[args.steps - 1])
@@ -712,6 +886,7 @@ def bench_headline(hip, args, F):
             verified &= (got.row_ptr.tobytes() == rp.tobytes()
                          and got.col_idx.tobytes() == ci.tobytes()
                          and got.values.tobytes() == val.tobytes())
+        fp64 = oracle.merge_f64_errors([oracle.f64_error(last, last, rp, ci, got.values)])
 
     ops = rvsp.Descriptor(memory=F.MEMORY_DEVICE)
     ops.estimate(dA, dA)
@@ -787,6 +962,7 @@ def bench_headline(hip, args, F):
                       "first_compute_fill_numeric": round(head["t_first"] * 1e3, 3),
                       "autotune": round(head["t_tune"] * 1e3, 1) if head["t_tune"] else None},
         "verified_bit_exact": verified,
+        "fp64_error": fp64,
     }
     for r in runs.values():
         r["C"].free()
@@ -808,15 +984,19 @@ def main():
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--only", choices=["fixed", "tuned"], default=None,
                     help="time one plan only (profiling runs)")
-    ap.add_argument("--extra", default="batched,powerlaw,banded",
+    ap.add_argument("--extra", default="uniform,batched,powerlaw,banded",
                     help="secondary workloads reported under 'configs' (comma list from "
-                         "batched, powerlaw, banded, legacy; or none)")
+                         "uniform, batched, powerlaw, banded, legacy; or none)")
     ap.add_argument("--only-extra", action="store_true",
                     help="skip the headline workload (profiling runs of the extras)")
     ap.add_argument("--pairs", type=int, default=2048)
+    ap.add_argument("--uniform-pairs", type=int, default=64)
+    ap.add_argument("--uniform-runs", type=int, default=5)
+    ap.add_argument("--uniform-graph-reps", type=int, default=50)
     ap.add_argument("--batch-repeats", type=int, default=5)
     ap.add_argument("--batch-algo", choices=["fixed", "learned"], default="learned")
     ap.add_argument("--rmat-scale", type=int, default=20)
+    ap.add_argument("--banded-runs", type=int, default=3)
     ap.add_argument("--powerlaw-runs", type=int, default=3)
     ap.add_argument("--powerlaw-plans", default=None,
                     help="comma list of plain_fixed, batched_fixed, learned_batched "
@@ -852,7 +1032,7 @@ def main():
 
     hip = Hip()
     out = {} if args.only_extra else bench_headline(hip, args, F)
-    extras = {"batched": bench_batched, "powerlaw": bench_powerlaw, "banded": bench_banded,
+    extras = {"uniform": bench_uniform, "batched": bench_batched, "powerlaw": bench_powerlaw, "banded": bench_banded,
               "legacy": bench_legacy}
     configs = {}
     for name in extra:

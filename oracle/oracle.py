@@ -161,6 +161,36 @@ def spgemm_f64(A, B, rp, ci, r0=0):
     return v[:int(rp[-1])], mag[:int(rp[-1])]
 
 
+def f64_error(A, B, rp, ci, got, r0=0):
+    """Error of FP32 values `got` on the structure (rp, ci) of rows
+    [r0, ...) against this FP64 oracle: the largest |c - c64| / |c64| over
+    entries whose |c64| is at least 1e-6 of their sum of |a*b| (an entry
+    cancelled that far has no relative scale; such entries are counted), and
+    the condition-aware max |c - c64| / sum_k |a_ik b_kj| over all entries.
+    The reference's own validation (bench/bench.c:300-304) compares FP32
+    results with 1e-5 absolute + 1e-4 relative slack; the FP64 figures here
+    are reported against a 1e-5 tolerance."""
+    v64, mag = spgemm_f64(A, B, np.asarray(rp, dtype=np.int32), ci, r0)
+    d = np.abs(np.asarray(got, dtype=np.float64) - v64)
+    scaled = np.abs(v64) >= 1e-6 * mag
+    rel = float((d[scaled] / np.abs(v64[scaled])).max()) if scaled.any() else 0.0
+    cond = float((d / np.where(mag > 0, mag, 1.0)).max()) if d.size else 0.0
+    return {"entries": int(d.size), "max_rel_err": rel, "max_cond_err": cond,
+            "cancelled_entries": int((~scaled).sum())}
+
+
+def merge_f64_errors(errs, tolerance=1e-5):
+    """Combines f64_error results of several samples (None if all empty)."""
+    errs = [e for e in errs if e and e["entries"]]
+    if not errs:
+        return None
+    return {"entries": sum(e["entries"] for e in errs),
+            "max_rel_err": max(e["max_rel_err"] for e in errs),
+            "max_cond_err": max(e["max_cond_err"] for e in errs),
+            "cancelled_entries": sum(e["cancelled_entries"] for e in errs),
+            "tolerance": tolerance}
+
+
 # ---------------------------------------------------------------------------
 # The reference library built from its own sources.
 

@@ -110,6 +110,35 @@ def test_oracle_f64_matches_dense_numpy(seed):
     assert np.all(np.abs(v32 - v64) <= 1e-6 * np.maximum(mag, 1e-30))
 
 
+def test_f64_error_report_against_dense_numpy():
+    # The report's two figures recomputed from a dense float64 product and
+    # |A| |B|; then a planted error of known size must show up exactly.
+    rng = np.random.Generator(np.random.PCG64(77))
+    A = Csr.from_dense(np.where(rng.random((40, 30)) < 0.3, rng.uniform(-1, 1, (40, 30)), 0))
+    B = Csr.from_dense(np.where(rng.random((30, 50)) < 0.3, rng.uniform(-1, 1, (30, 50)), 0))
+    rp, ci, v32 = oracle.spgemm(A, B)
+    r = np.repeat(np.arange(A.rows), np.diff(rp))
+    Ad, Bd = A.to_dense().astype(np.float64), B.to_dense().astype(np.float64)
+    exact = (Ad @ Bd)[r, ci]
+    mag = (np.abs(Ad) @ np.abs(Bd))[r, ci]
+    e = oracle.f64_error(A, B, rp, ci, v32)
+    d = np.abs(v32.astype(np.float64) - exact)
+    assert e["entries"] == ci.size
+    np.testing.assert_allclose(e["max_cond_err"], (d / mag).max(), rtol=1e-6, atol=1e-15)
+    keep = np.abs(exact) >= 1e-6 * mag
+    assert e["cancelled_entries"] == int((~keep).sum())
+    np.testing.assert_allclose(e["max_rel_err"], (d[keep] / np.abs(exact[keep])).max(),
+                               rtol=1e-6, atol=1e-15)
+    assert e["max_cond_err"] < 1e-6
+    bad = v32.copy()
+    k = int(np.argmax(mag))
+    bad[k] = np.float32(exact[k] + 1e-3 * mag[k])
+    e2 = oracle.f64_error(A, B, rp, ci, bad)
+    np.testing.assert_allclose(e2["max_cond_err"], 1e-3, rtol=1e-3)
+    m = oracle.merge_f64_errors([e, e2, None])
+    assert m["entries"] == 2 * ci.size and m["max_cond_err"] == e2["max_cond_err"]
+
+
 @pytest.mark.parametrize("variant", [1, 2])
 def test_planted_mistakes_are_caught(variant):
     # A fused multiply-add (1) or a reversed A-row order (2) must change the
