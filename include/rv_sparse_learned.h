@@ -14,6 +14,11 @@
  * the chosen table cannot hold a bin's largest row, the bin runs on the
  * global dense accumulator. Output bits never depend on the network.
  *
+ * The last input feature says whether the choice is for numeric-only
+ * computes that reuse an analysed structure (the first such compute of a
+ * HIP_LEARNED descriptor selects its numeric bins again with it set, and may
+ * then pick the position-map classes, which pay off only over repeats).
+ *
  * Training data comes from the library itself: rvsp_spgemm_autotune times
  * every candidate shape of every non-empty bin and appends one sample per bin
  * (features, time per class) to a process-wide sample store. rvsp_learned_fit
@@ -33,9 +38,9 @@ extern "C" {
 #endif
 
 /* Network dimensions: features -> hidden (ReLU) -> hidden (ReLU) -> classes. */
-#define RVSP_MLP_FEATURES 16
+#define RVSP_MLP_FEATURES 17
 #define RVSP_MLP_HIDDEN 64
-#define RVSP_MLP_CLASSES 52
+#define RVSP_MLP_CLASSES 56
 /* Parameter count. Layout, row-major: W1[H][F], b1[H], W2[H][H], b2[H],
  * W3[K][H], b3[K]. */
 #define RVSP_MLP_PARAMS                                                   \
@@ -44,13 +49,16 @@ extern "C" {
      RVSP_MLP_CLASSES * RVSP_MLP_HIDDEN + RVSP_MLP_CLASSES)
 
 /* What one output class of the network stands for. cap_mult scales the bin's
- * base LDS table size (hash classes: 1 or 2; others 1). */
+ * base LDS table size (hash classes: 1 or 2; others 1); cap is the tile
+ * size in positions of the long-row classes (RVSP_KIND_BLOCK; variant 0:
+ * workgroup tiles, variant 1: wave tiles), 0 for the others. */
 typedef struct
 {
     int32_t kind;     /* rvsp_kind_t */
     int32_t group;    /* lanes per row */
     int32_t cap_mult;
     int32_t variant;
+    int32_t cap;
 } rvsp_learned_class_t;
 
 /* RVSP_ERROR_INVALID_ARGUMENT when cls is outside [0, RVSP_MLP_CLASSES). */

@@ -19,7 +19,7 @@ W1[H][F], b1[H], W2[H][H], b2[H], W3[K][H], b3[K].
 """
 import numpy as np
 
-F, H, K = 16, 64, 52
+F, H, K = 17, 64, 56
 P = H * F + H + H * H + H + K * H + K
 
 

@@ -704,6 +704,13 @@ rvsp_status_t promote_to_map(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr
     }
   }
   if (!any) return RVSP_SUCCESS;
+  if (d->algo == RVSP_SPGEMM_ALGO_HIP_LEARNED) {
+    // The network picks each moved bin's map shape (lanes, variant) with its
+    // reuse input set; apply_choice still vetoes what cannot hold the rows.
+    std::vector<BinFeatures> f;
+    RVSP_TRY(bin_features(d, d->numeric, PHASE_NUMERIC, A.rp, (const int*)d->c_rp.p, s, &f));
+    RVSP_TRY(select_learned(d, t, PHASE_NUMERIC, f, s, true));
+  }
   RVSP_TRY(plan_map(d, s));  // bins over the memory budget go back to SEARCH
   if (d->map_width) RVSP_TRY(ensure_map_built(d, A, B, c_ci, s));
   return RVSP_SUCCESS;

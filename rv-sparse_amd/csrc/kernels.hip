@@ -2022,6 +2022,267 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Long rows, wave-independent tiles (KIND_BLOCK variant 1).
+//
+// Every wave (one per 64-thread workgroup) owns a run of consecutive output
+// positions of one row and walks it in tiles of up to Tw positions, its
+// accumulators in LDS. For each tile it streams the row's products that
+// land in the tile's columns in reference order (A entries ascending, each
+// B row's positions ascending), 64 at a time: per chunk of 64 A entries
+// every lane narrows its entry's B row to the tile's columns (cursor along
+// the run, or a search through the split table), a wave scan lays the
+// entries' products end to end, and the entry of every product of a step is
+// the running maximum of the entries that start in the step (one LDS mark
+// per entry). Products of one entry have distinct columns, so a step drawn
+// from a single entry updates its accumulators in one go; a step that spans
+// several entries applies products on the same position lowest lane first
+// (per round an LDS atomicMin of (position, lane) on a small per-wave hash
+// of positions). A wave's LDS operations execute in program order and no
+// other wave touches its LDS, so the kernel has no barrier at all. A tile's
+// positions are found through a bitmap index of its column span (32 Tw
+// columns at most) or, for sparser tiles, an LDS hash of its columns.
+constexpr int kWaveTags = 128;   // per-wave hash of pending positions
+constexpr int kWaveItems = 4;    // 64-product sub-steps per step
+constexpr int kWaveCur = 16;     // cursor chunks: A rows up to 1024 entries
+constexpr int kWavePosBits = 11; // positions per tile < 2^11
+constexpr int kWaveHashSpan = 1 << (31 - kWavePosBits);  // hash tiles' column span
+
+int wave_tile_lds_bytes(int Tw) {
+  // vals[Tw] | index (Tw uint2 words, or 2 Tw hash slots) | tags | marks
+  return (3 * Tw + kWaveTags + kWave) * 4;
+}
+
+// Keeps the compiler from moving LDS accesses across this point (the
+// hardware already executes one wave's LDS operations in program order), so
+// values written by other lanes of the wave are read from LDS.
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+template <int G>
+__device__ __forceinline__ int group_incl_max(int v) {
+  const int l = threadIdx.x & (G - 1);
+#pragma unroll
+  for (int d = 1; d < G; d <<= 1) {
+    const int t = __shfl_up(v, d, G);
+    if (l >= d) v = max(v, t);
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(kWave) void k_numeric_wave(
+    const int* __restrict__ rows, int nrows, int Tw, DevCsr A, DevCsr B,
+    const int* __restrict__ c_rp, const int* __restrict__ c_ci,
+    float* __restrict__ c_val, SplitTable split) {
+  extern __shared__ __attribute__((aligned(16))) int wave_lds[];
+  float* vals = reinterpret_cast<float*>(wave_lds);
+  uint2* idx = reinterpret_cast<uint2*>(wave_lds + Tw);  // Tw is even: 8-byte aligned
+  int* hsh = wave_lds + Tw;                              // 2 Tw slots, same bytes as idx
+  int* tags = wave_lds + 3 * Tw;
+  int* mark = tags + kWaveTags;
+  const int lane = threadIdx.x;
+  const int hmask = 2 * Tw - 1;         // Tw is a power of two
+  const int hbits = 31 - __clz(2 * Tw);  // 2 Tw == 1 << hbits
+  for (int i = lane; i < kWaveTags; i += kWave) tags[i] = INT_MAX;
+  for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
+    const int row = rows[r];
+    const int c_s = c_rp[row];
+    const int n = c_rp[row + 1] - c_s;
+    const int runs = min((int)gridDim.y, n / Tw + (n % Tw != 0));
+    if ((int)blockIdx.y >= runs) continue;
+    const int rq = n / runs, rr = n % runs, y = blockIdx.y;
+    const int P0 = rq * y + rr * y / runs;
+    const int P1 = rq * (y + 1) + rr * (y + 1) / runs;
+    const int a_s = A.rp[row], a_e = A.rp[row + 1];
+    const bool cursors = a_e - a_s <= kWaveCur * kWave;
+    int cur[kWaveCur];
+    bool first = true;
+    for (int p0 = P0; p0 < P1;) {
+      int pe = min(P1, p0 + Tw);
+      const int clo = c_ci[c_s + p0];
+      if (c_ci[c_s + pe - 1] - clo >= kWaveHashSpan) {
+        // Too wide even for the hash's column offsets: fewer positions.
+        int lo = p0 + 1, hi = pe;  // answer in [lo, hi]: first past the span
+        while (lo < hi) {
+          const int m = (lo + hi) >> 1;
+          if (c_ci[c_s + m] - clo < kWaveHashSpan) lo = m + 1; else hi = m;
+        }
+        pe = lo;
+      }
+      const int tn = pe - p0;
+      const int chi = c_ci[c_s + pe - 1];
+      const bool use_bm = chi - clo < 32 * Tw;  // uniform
+      const bool narrow = !(p0 == 0 && pe == n);
+      const int iw = ((chi - clo) >> 5) + 1;
+      if (use_bm) {
+        for (int w = lane; w < iw; w += kWave) idx[w] = make_uint2(0u, 0u);
+      } else {
+        for (int w = lane; w <= hmask; w += kWave) hsh[w] = -1;
+      }
+      for (int j = lane; j < tn; j += kWave) vals[j] = 0.0f;
+      if (use_bm) {
+        for (int j = lane; j < tn; j += kWave) {
+          const int d = c_ci[c_s + p0 + j] - clo;
+          atomicOr(&idx[d >> 5].x, 1u << (d & 31));
+        }
+        // Positions before each word: per lane a run of consecutive words.
+        const int per = (iw + kWave - 1) / kWave;
+        const int w_beg = min(iw, lane * per), w_end = min(iw, w_beg + per);
+        int cnt = 0;
+        for (int w = w_beg; w < w_end; ++w) cnt += __popc(idx[w].x);
+        int run = group_incl_scan<kWave>(cnt) - cnt;
+        for (int w = w_beg; w < w_end; ++w) {
+          idx[w].y = (unsigned)run;
+          run += __popc(idx[w].x);
+        }
+      } else {
+        for (int j = lane; j < tn; j += kWave) {
+          const int d = c_ci[c_s + p0 + j] - clo;
+          const int key = (d << kWavePosBits) | j;
+          int slot = hash_slot(d, hbits);
+          while (atomicCAS(&hsh[slot], -1, key) != -1) slot = (slot + 1) & hmask;
+        }
+      }
+      int chunk = 0;
+      for (int cb = a_s; cb < a_e; cb += kWave, ++chunk) {
+        const int ap = cb + lane;
+        int bs = 0, len = 0;
+        float a = 0.0f;
+        if (ap < a_e) {
+          const int k = A.ci[ap];
+          a = A.val[ap];
+          int be = B.rp[k + 1];
+          if (narrow) {
+            const int b0 = B.rp[k];
+            int so = -1;
+            if (split.off != nullptr && be - b0 >= kSplitMin && split.off[k + 1] > split.off[k])
+              so = split.off[k];
+            int2 meta = make_int2(0, 0);
+            if (so >= 0) meta = *reinterpret_cast<const int2*>(split.sp + so);
+            int lo = 0;
+            if (cursors && !first) {
+#pragma unroll
+              for (int c = 0; c < kWaveCur; ++c)
+                if (c == chunk) lo = cur[c];
+            } else if (so >= 0) {
+              lo = split_lower_bound(B.ci, split.sp + so, meta, b0, be, b0, clo);
+            } else {
+              lo = gm_lower_bound(B.ci, b0, be, clo);
+            }
+            bs = lo;
+            if (so >= 0 && !(cursors && !first))
+              be = split_lower_bound(B.ci, split.sp + so, meta, b0, be, lo, chi + 1);
+            else
+              be = gm_gallop(B.ci, lo, be, chi + 1);
+            if (cursors) {
+#pragma unroll
+              for (int c = 0; c < kWaveCur; ++c)
+                if (c == chunk) cur[c] = be;
+            }
+          } else {
+            bs = B.rp[k];
+          }
+          len = be - bs;
+        }
+        const int incl = group_incl_scan<kWave>(len);
+        const int excl = incl - len;
+        const int total = __shfl(incl, kWave - 1);
+        int carry = -1;  // entry of the previous step's last product
+        // The entry, B position and scale of product q0 + lane.
+        auto locate = [&](int q0, int& e, int& bi, float& av) {
+          mark[lane] = -1;
+          if (len > 0 && excl >= q0 && excl < q0 + kWave) mark[excl - q0] = lane;
+          wave_lds_order();  // other lanes' marks are read below
+          const int m = mark[lane];
+          wave_lds_order();  // ... before the next step clears them
+          e = max(group_incl_max<kWave>(m), carry);
+          carry = __shfl(e, kWave - 1);
+          bi = __shfl(bs, e) + (q0 + lane - __shfl(excl, e));
+          av = __shfl(a, e);
+        };
+        // Steps of kWaveItems x 64 products; the next step's gathers are
+        // issued before this step's lookups and adds, so up to 2 kWaveItems
+        // products per lane are in flight.
+        int e[kWaveItems], col[kWaveItems], ne[kWaveItems], ncol[kWaveItems];
+        float av[kWaveItems], bv[kWaveItems], nav[kWaveItems], nbv[kWaveItems];
+        auto fetch = [&](int q0, int* fe, int* fc, float* fa, float* fb) {
+#pragma unroll
+          for (int i = 0; i < kWaveItems; ++i) {
+            const int q = q0 + i * kWave;
+            int bi = 0;
+            fe[i] = carry;
+            fc[i] = -1;
+            fa[i] = fb[i] = 0.0f;
+            if (q < total) {  // uniform
+              locate(q, fe[i], bi, fa[i]);
+              if (q + lane < total) {
+                fc[i] = B.ci[bi];
+                fb[i] = B.val[bi];
+              }
+            }
+          }
+        };
+        fetch(0, e, col, av, bv);
+        for (int q0 = 0; q0 < total; q0 += kWaveItems * kWave) {
+          if (q0 + kWaveItems * kWave < total) fetch(q0 + kWaveItems * kWave, ne, ncol, nav, nbv);
+#pragma unroll
+          for (int i = 0; i < kWaveItems; ++i) {
+            const int qs = q0 + i * kWave;
+            if (qs >= total) break;  // uniform
+            const bool valid = qs + lane < total;
+            int pos = -1;
+            float prod = 0.0f;
+            if (valid) {
+              const int d = col[i] - clo;
+              if (use_bm) {
+                const uint2 v = idx[d >> 5];
+                pos = (int)v.y + __popc(v.x & ((1u << (d & 31)) - 1u));
+              } else {
+                int slot = hash_slot(d, hbits);
+                int v = hsh[slot];
+                while ((v >> kWavePosBits) != d) {
+                  slot = (slot + 1) & hmask;
+                  v = hsh[slot];
+                }
+                pos = v & ((1 << kWavePosBits) - 1);
+              }
+              prod = av[i] * bv[i];  // rounded here, added below
+            }
+            const int last = min(kWave, total - qs) - 1;
+            if (__shfl(e[i], 0) == __shfl(e[i], last)) {
+              // One entry: distinct positions.
+              if (valid) vals[pos] = vals[pos] + prod;
+            } else {
+              bool act = valid;
+              const int key = (pos << 6) | lane;
+              const int slot = (int)(((unsigned)pos * 2654435761u) >> 25);  // kWaveTags = 128
+              while (__any(act)) {
+                if (act) atomicMin(&tags[slot], key);
+                if (act && tags[slot] == key) {
+                  vals[pos] = vals[pos] + prod;
+                  tags[slot] = INT_MAX;
+                  act = false;
+                }
+              }
+            }
+          }
+#pragma unroll
+          for (int i = 0; i < kWaveItems; ++i) {
+            e[i] = ne[i];
+            col[i] = ncol[i];
+            av[i] = nav[i];
+            bv[i] = nbv[i];
+          }
+        }
+      }
+      for (int j = lane; j < tn; j += kWave) c_val[c_s + p0 + j] = vals[j];
+      p0 = pe;
+      first = false;
+    }
+  }
+}
+
 int tiled_lds_bytes(int T) {
   return kTiledItems * kTiledWaves * kTiledWaves * 4 + kTiledGroup * 8 +
          tiled_index_words(T) * 8 + ((T + 1) & ~1) * 4 + kTiledWaves * kTagSlots * 4 +
@@ -2072,6 +2333,13 @@ hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
       const long long most = max_row > 0 ? max_row : 1;
       const int ytiles = (int)std::min<long long>(16, (most + b.cap - 1) / b.cap);
       const int gx = (std::min(nrows, 1 << 20) + 7) / 8 * 8;
+      if (b.variant == 1) {
+        // Wave tiles: up to kWaveRuns runs per row, one wave each.
+        const int wy = (int)std::min<long long>(kWaveRuns, (most + b.cap - 1) / b.cap);
+        hipLaunchKernelGGL(k_numeric_wave, dim3(gx, wy), dim3(kWave), b.lds_bytes, s,
+                           rows, nrows, b.cap, A, B, c_rp, c_ci, c_val, split);
+        return hipGetLastError();
+      }
       hipLaunchKernelGGL(k_numeric_tiled, dim3(gx, ytiles), dim3(kBlock), b.lds_bytes, s,
                          rows, nrows, b.cap, A, B, c_rp, c_ci, c_val, split);
       return hipGetLastError();
@@ -2200,7 +2468,7 @@ hipError_t init_kernel_attributes(int lds_bytes) {
   RVSP_ALLOW((k_bitmap_row<false, kBitmapCount, kWideBlock>))
   RVSP_ALLOW((k_bitmap_row<false, kBitmapFill, kWideBlock>))
   RVSP_ALLOW((k_bitmap_row<false, kBitmapCountKeep, kWideBlock>))
-  RVSP_ALLOW(k_numeric_rowblock<false>) RVSP_ALLOW(k_numeric_tiled)
+  RVSP_ALLOW(k_numeric_rowblock<false>) RVSP_ALLOW(k_numeric_tiled) RVSP_ALLOW(k_numeric_wave)
 #define RVSP_ALLOW_MAP(G)                                                     \
   RVSP_ALLOW((k_numeric_map<G, unsigned char, 0>))                             \
   RVSP_ALLOW((k_numeric_map<G, unsigned char, 1>))                             \

@@ -199,6 +199,12 @@ hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
 int tiled_lds_bytes(int T);
 // Default tile of the long-row numeric kernel: three workgroups fit a CU.
 constexpr int kTiledPositions = 6144;
+// Dynamic LDS of the wave-tile long-row numeric kernel (KIND_BLOCK variant
+// 1) for tiles of Tw positions (a power of two, 256..2048); a workgroup is
+// one wave, and a row gets up to kWaveRuns runs of tiles.
+int wave_tile_lds_bytes(int Tw);
+constexpr int kWaveRuns = 64;
+constexpr int kWavePositions = 1024;
 
 // --- position map (numeric structure cache) -------------------------------
 struct MapView {

@@ -41,16 +41,31 @@ constexpr int kClsBitmap = 14;
 constexpr int kClsGlobal = 15;
 constexpr int kClsSearch = 16;   // 7 lane counts
 constexpr int kClsMap = 23;      // 3 variants x 7 lane counts
-constexpr int kClsBlock = 44;
+constexpr int kClsBlock = 44;    // workgroup tiles of kTiledPositions
 constexpr int kClsDirect = 45;   // 7 lane counts
-static_assert(kClsDirect + 7 == kMlpK, "class table size");
+constexpr int kClsBlockMore = 52;  // 4 more long-row shapes (kBlockMore)
+static_assert(kClsBlockMore + 4 == kMlpK, "class table size");
+
+// Long-row classes beyond kClsBlock: (variant, positions per tile).
+// Workgroup tiles of 4096 and 3072 positions fit four CUs' worth of
+// workgroups more per CU than the default; wave tiles trade more narrowing
+// for no barriers.
+const int kBlockMore[4][2] = {{0, 4096}, {0, 3072}, {1, 512}, {1, 1024}};
+
+// Index of the reuse flag among the network's inputs.
+constexpr int kReuseFeature = 16;
 
 rvsp_learned_class_t class_entry(int cls) {
   rvsp_learned_class_t c;
   c.cap_mult = 1;
   c.variant = 0;
   c.group = kBlock;
-  if (cls < kClsBitmap) {
+  c.cap = 0;
+  if (cls >= kClsBlockMore) {
+    c.kind = KIND_BLOCK;
+    c.variant = kBlockMore[cls - kClsBlockMore][0];
+    c.cap = kBlockMore[cls - kClsBlockMore][1];
+  } else if (cls < kClsBitmap) {
     c.kind = KIND_HASH;
     c.group = kGroups[cls % 7];
     c.cap_mult = cls < kClsHash2 ? 1 : 2;
@@ -67,6 +82,7 @@ rvsp_learned_class_t class_entry(int cls) {
     c.variant = (cls - kClsMap) / 7;
   } else if (cls == kClsBlock) {
     c.kind = KIND_BLOCK;
+    c.cap = kTiledPositions;
   } else {
     c.kind = KIND_DIRECT;
     c.group = kGroups[cls - kClsDirect];
@@ -109,7 +125,10 @@ int fixed_class(int phase, int b) {
     case KIND_SEARCH: return gi < 0 ? -1 : kClsSearch + gi;
     case KIND_MAP: return gi < 0 ? -1 : kClsMap + 7 * s.variant + gi;
     case KIND_BITMAP: return kClsBitmap;
-    case KIND_BLOCK: return kClsBlock;
+    case KIND_BLOCK:
+      for (int i = 0; i < 4; ++i)
+        if (s.variant == kBlockMore[i][0] && s.cap == kBlockMore[i][1]) return kClsBlockMore + i;
+      return kClsBlock;
     case KIND_GLOBAL: return kClsGlobal;
     default: return -1;
   }
@@ -117,13 +136,22 @@ int fixed_class(int phase, int b) {
 
 uint64_t candidate_classes(int phase, int b, bool map_ok);
 
+uint64_t map_classes() {
+  uint64_t m = 0;
+  for (int c = kClsMap; c < kClsBlock; ++c) m |= 1ull << c;
+  return m;
+}
+
 // The classes the network may pick for bin b (a subset of the tuner's).
-uint64_t selectable_classes(int phase, int b) {
-  // The position map pays off only over repeated numeric-only computes,
-  // which the features cannot see, so its classes are left to the tuner;
-  // the global dense accumulator stays the veto's fallback for the short-row
-  // bins (its cost there grows with B's column count, which the features do
-  // not carry) and a choice for the long-row ones.
+uint64_t selectable_classes(int phase, int b, bool reuse) {
+  // The position map pays off only over repeated numeric-only computes: its
+  // classes are selectable only for reuse (the reuse input set), and then
+  // they are the only ones, since the descriptor's first numeric-only
+  // compute moves exactly the bins that can take a map. The global dense
+  // accumulator stays the veto's fallback for the short-row bins (its cost
+  // there grows with B's column count, which the features do not carry)
+  // and a choice for the long-row ones.
+  if (reuse) return phase == PHASE_NUMERIC ? candidate_classes(phase, b, true) & map_classes() : 0;
   uint64_t m = candidate_classes(phase, b, false);
   const int kind = fixed_table(phase).spec[b].kind;
   if (kind == KIND_HASH || kind == KIND_SEARCH || kind == KIND_DIRECT)
@@ -151,6 +179,7 @@ uint64_t candidate_classes(int phase, int b, bool map_ok) {
     add(kClsSearch, 7);
     if (map_ok) add(kClsMap, 21);
     add(kClsBlock, 1);
+    add(kClsBlockMore, 4);
   }
   add(kClsGlobal, 1);
   return m;
@@ -175,7 +204,7 @@ BinChoice class_choice(int cls, int phase, int b, int max_metric) {
       ch.cap = fs.kind == KIND_SEARCH ? fs.cap : next_pow2(std::max(16LL, need));
       break;
     case KIND_BLOCK:
-      ch.cap = kTiledPositions;  // positions per tile
+      ch.cap = c.cap;  // positions per tile
       break;
     case KIND_BITMAP:
       ch.cap = fs.cap;
@@ -377,14 +406,18 @@ rvsp_status_t train_device(float* params, const float* x, const float* q,
 }
 
 // Soft targets from measured times: q_c ~ exp(-ln(t_c / t_best) / tau) over
-// the candidates of the sample that selection may pick (the position-map
-// classes are left to the tuner, as in select_learned).
+// the candidates of the sample that selection may pick (for a reuse row, the
+// position-map classes; otherwise every other class, as in select_learned).
 constexpr double kTau = 0.1;
+
+bool reuse_row(const rvsp_learned_sample_t& smp) {
+  return smp.features[kReuseFeature] > 0.5f;
+}
 
 void sample_targets(const rvsp_learned_sample_t& smp, float* q, unsigned long long* mask) {
   double best = 0;
   *mask = 0;
-  const uint64_t sel = selectable_classes(smp.phase, smp.bin);
+  const uint64_t sel = selectable_classes(smp.phase, smp.bin, reuse_row(smp));
   for (int c = 0; c < kMlpK; ++c)
     if (smp.us[c] >= 0.f && ((sel >> c) & 1ull)) {
       *mask |= 1ull << c;
@@ -412,7 +445,7 @@ void record_sample(const rvsp_learned_sample_t& s) {
 // Selection (work_estimation with RVSP_SPGEMM_ALGO_HIP_LEARNED).
 
 rvsp_status_t select_learned(rvsp_spgemm_descr* d, PhaseTable& t, int phase,
-                             const std::vector<BinFeatures>& f, hipStream_t s) {
+                             const std::vector<BinFeatures>& f, hipStream_t s, bool reuse) {
   // The model is copied out under the lock; the device work below runs on
   // the descriptor's own scratch without it, so concurrent estimations (a
   // batch's lanes) never wait for each other's streams.
@@ -435,7 +468,9 @@ rvsp_status_t select_learned(rvsp_spgemm_descr* d, PhaseTable& t, int phase,
   std::vector<unsigned long long> mask;
   for (int b = 1; b < t.rule.nbins && b < (int)f.size(); ++b) {
     if (f[b].rows <= 0) continue;
-    uint64_t m = selectable_classes(phase, b);
+    // Reuse selection only re-shapes the bins already moved to the map.
+    if (reuse && t.spec[b].kind != KIND_MAP) continue;
+    uint64_t m = selectable_classes(phase, b, reuse);
     for (int c = 0; c < kMlpK; ++c) {
       if (!((m >> c) & 1ull)) continue;
       PhaseTable trial = t;
@@ -447,6 +482,7 @@ rvsp_status_t select_learned(rvsp_spgemm_descr* d, PhaseTable& t, int phase,
     if (!m) continue;
     float raw[kMlpF];
     bin_feature_vector(phase, b, t.rule.bound[b], f[b], raw);
+    raw[kReuseFeature] = reuse ? 1.f : 0.f;
     float nx[kMlpF];
     normalise(raw, norm, nx);
     bins.push_back(b);
@@ -542,6 +578,21 @@ rvsp_status_t rvsp_learned_fit(int32_t first, int32_t count, int32_t epochs, flo
     if (first < 0 || count < 1 || (int64_t)first + count > (int64_t)g_samples.size())
       return RVSP_ERROR_INVALID_ARGUMENT;
     smp.assign(g_samples.begin() + first, g_samples.begin() + first + count);
+  }
+  // A numeric sample with timed position-map classes also teaches the reuse
+  // choice: a copy with the reuse input set, whose targets are the map
+  // classes' times (the tuner times each map class per numeric-only compute).
+  const size_t base_n = smp.size();
+  for (size_t i = 0; i < base_n; ++i) {
+    if (smp[i].phase != PHASE_NUMERIC || reuse_row(smp[i])) continue;
+    const uint64_t maps = map_classes();
+    bool timed = false;
+    for (int c = 0; c < kMlpK; ++c)
+      if (((maps >> c) & 1ull) && smp[i].us[c] >= 0.f) timed = true;
+    if (!timed) continue;
+    rvsp_learned_sample_t r = smp[i];
+    r.features[kReuseFeature] = 1.f;
+    smp.push_back(r);
   }
   const int n = (int)smp.size();
   Model m;

@@ -84,8 +84,14 @@ int lds_bytes_for(int phase, int kind, int group, int cap, int variant) {
       // Tiles of up to `cap` positions: accumulators, the column index,
       // staging of one group of products, per-wave order tags and one chunk
       // of A-entry metadata.
-      if (phase != PHASE_NUMERIC || cap < kBlock) break;
-      bytes = tiled_lds_bytes(cap);
+      if (phase != PHASE_NUMERIC) break;
+      if (variant == 1) {
+        // Wave tiles: accumulators, index or hash, tags, entry marks.
+        if (cap < 256 || cap > 2048 || (cap & (cap - 1))) break;
+        bytes = wave_tile_lds_bytes(cap);
+      } else if (variant == 0 && cap >= kBlock) {
+        bytes = tiled_lds_bytes(cap);
+      }
       break;
     default:
       break;

@@ -28,8 +28,11 @@ void select_tuned(PhaseTable& t, int phase, const std::vector<BinFeatures>& f);
 // HIP_LEARNED: the MLP selector's choice for every non-empty bin.
 // The model is read under the library's lock; the prediction runs on the
 // descriptor's scratch (learn_feat) with no lock held.
+// reuse: selection for numeric-only computes of an analysed structure; it
+// re-shapes only the bins already on the position map, among map classes.
 rvsp_status_t select_learned(rvsp_spgemm_descr* d, PhaseTable& t, int phase,
-                             const std::vector<BinFeatures>& f, hipStream_t s);
+                             const std::vector<BinFeatures>& f, hipStream_t s,
+                             bool reuse = false);
 
 // Output classes of the selector network (rv_sparse_learned.h). A class is a
 // kernel shape relative to the fixed table's bin: its kind, lanes per row and

@@ -67,14 +67,14 @@ class BinChoice(C.Structure):
 
 
 # include/rv_sparse_learned.h
-MLP_FEATURES, MLP_HIDDEN, MLP_CLASSES = 16, 64, 52
+MLP_FEATURES, MLP_HIDDEN, MLP_CLASSES = 17, 64, 56
 MLP_PARAMS = (MLP_HIDDEN * MLP_FEATURES + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN
               + MLP_CLASSES * MLP_HIDDEN + MLP_CLASSES)
 
 
 class LearnedClass(C.Structure):
     _fields_ = [("kind", C.c_int32), ("group", C.c_int32), ("cap_mult", C.c_int32),
-                ("variant", C.c_int32)]
+                ("variant", C.c_int32), ("cap", C.c_int32)]
 
 
 class LearnedSample(C.Structure):

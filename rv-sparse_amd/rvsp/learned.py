@@ -39,8 +39,13 @@ def class_table():
         e = F.LearnedClass()
         check(lib().rvsp_learned_class_info(c, C.byref(e)), "class_info")
         out.append(dict(kind=e.kind, group=e.group, cap_mult=e.cap_mult, variant=e.variant,
-                        name=f"{F.KIND_NAMES[e.kind]}/g{e.group}x{e.cap_mult}v{e.variant}"))
+                        cap=e.cap,
+                        name=f"{F.KIND_NAMES[e.kind]}/g{e.group}x{e.cap_mult}v{e.variant}"
+                             + (f"t{e.cap}" if e.cap else "")))
     return out
+
+
+REUSE = NF - 1  # the network input that marks a numeric-only reuse choice
 
 
 def map_class_mask():
@@ -100,18 +105,44 @@ def load_samples(path):
         return {k: z[k] for k in z.files}
 
 
-def candidate_masks(sm, include_map=False):
-    """Per-sample bit sets of the measured classes the network may pick (as
-    the library's selection: no position-map classes, and no global dense
-    accumulator for bins whose fixed kind is a short-row kernel)."""
+def reuse_rows(sm):
+    """The numeric samples with timed position-map classes as the library's
+    training adds them (rvsp_learned_fit): the reuse input set, and as the
+    plain choice the class the plain promotion uses (the fixed search bin's
+    lanes, or 64 lanes for a long-row bin, chunked variant)."""
     tab = class_table()
-    ok = sm["us"] >= 0
-    if not include_map:
-        ok = ok & ~np.array([(map_class_mask() >> c) & 1 for c in range(NK)], bool)[None, :]
+    maps = np.array([t["kind"] == F.KIND_MAP for t in tab], bool)
+    keep = (sm["phase"] == 2) & ((sm["us"] >= 0) & maps[None, :]).any(axis=1)
+    out = {k: np.array(v)[keep] for k, v in sm.items()}
+    out["features"] = out["features"].copy()
+    out["features"][:, REUSE] = 1.0
+    by = {(t["group"], t["variant"]): c for c, t in enumerate(tab) if t["kind"] == F.KIND_MAP}
+    fixed = []
+    for c in out["fixed_class"]:
+        t = tab[c]
+        g = t["group"] if t["kind"] == F.KIND_SEARCH else 64
+        fixed.append(by[(g, 2)])
+    out["fixed_class"] = np.array(fixed, np.int32)
+    return out
+
+
+def candidate_masks(sm, include_map=False):
+    """Per-sample bit sets of the measured classes the network may pick, as
+    the library's selection: for rows with the reuse input set, the
+    position-map classes only; otherwise no position-map classes (unless
+    include_map) and no global dense accumulator for bins whose fixed kind
+    is a short-row kernel."""
+    tab = class_table()
+    timed = sm["us"] >= 0
+    maps = np.array([t["kind"] == F.KIND_MAP for t in tab], bool)
+    glob = np.array([t["kind"] == F.KIND_GLOBAL for t in tab], bool)
     short = np.array([tab[c]["kind"] in (F.KIND_HASH, F.KIND_SEARCH, F.KIND_DIRECT)
                       for c in sm["fixed_class"]], bool)
-    glob = np.array([t["kind"] == F.KIND_GLOBAL for t in tab], bool)
-    ok = ok & ~(short[:, None] & glob[None, :])
+    reuse = sm["features"][:, REUSE] > 0.5
+    plain = timed & ~(short[:, None] & glob[None, :])
+    if not include_map:
+        plain = plain & ~maps[None, :]
+    ok = np.where(reuse[:, None], timed & maps[None, :], plain)
     return (ok.astype(np.uint64) << np.arange(NK, dtype=np.uint64)[None, :]).sum(
         axis=1, dtype=np.uint64)
 

@@ -9,10 +9,10 @@ the model (by default learned_model.txt next to librvsparse_hip.so, where
 RVSP_SPGEMM_ALGO_HIP_LEARNED descriptors find it) and the samples (.npz).
 
 Training set: R-MAT (a,b,c,d = .57,.19,.19,.05, edge factor 16) at scales
-14-17 with seeds 1..8 (seed 0 is kept for evaluation), plus lognormal-row
-products, uniform-k products, 5-point Laplacians and banded rows with a few
-dense rows, so that every bin of every phase is seen with short and long
-B rows.
+14-17 with seeds 1..8 and scale 18 with seeds 1..3 (seed 0 is kept for
+evaluation), plus lognormal-row products, uniform-k products, 5-point
+Laplacians and banded rows with a few dense rows at three sizes, so that
+every bin of every phase is seen with short and long B rows.
 """
 import argparse
 import os
@@ -30,6 +30,8 @@ def training_set():
     for scale in (14, 15, 16, 17):
         for seed in range(1, 9):
             yield f"rmat s{scale} seed{seed}", (lambda s=scale, k=seed: (gen.rmat(s, seed=k), None))
+    for seed in range(1, 4):
+        yield f"rmat s18 seed{seed}", (lambda k=seed: (gen.rmat(18, seed=k), None))
     for n in (4096, 32768, 131072):
         for seed in (1, 2):
             yield f"lognormal n{n} seed{seed}", (
@@ -41,6 +43,9 @@ def training_set():
         yield f"laplacian g{g}", (lambda g=g: (gen.laplacian_2d(g), None))
     yield "banded 50k", (lambda: gen.banded_mixed(rows=50_000, inner=20_000, cols=50_000,
                                                   seed=1))
+    for rows, seed in ((100_000, 2), (250_000, 3)):
+        yield f"banded {rows // 1000}k", (
+            lambda r=rows, k=seed: gen.banded_mixed(rows=r, inner=int(r * 0.4), cols=r, seed=k))
 
 
 def main(argv=None):

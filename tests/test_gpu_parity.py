@@ -100,6 +100,12 @@ PLAN_VARIANTS = {
     # so long B rows are narrowed and tiles loop.
     "numeric_block_t256": [(2, b, F.KIND_BLOCK, 256, 256) for b in range(2, 13)],
     "numeric_block_t1024": [(2, b, F.KIND_BLOCK, 256, 1024) for b in range(2, 13)],
+    # Wave tiles (KIND_BLOCK variant 1): 256-position tiles split rows into
+    # many runs and tiles (narrowing, cursors, hash-indexed sparse tiles);
+    # 2048 leaves most rows whole.
+    "numeric_wave_t256": [(2, b, F.KIND_BLOCK, 256, 256, 1) for b in range(2, 13)],
+    "numeric_wave_t1024": [(2, b, F.KIND_BLOCK, 256, 1024, 1) for b in range(2, 13)],
+    "numeric_wave_t2048": [(2, b, F.KIND_BLOCK, 256, 2048, 1) for b in range(2, 13)],
     "numeric_search_g1": [(2, b, F.KIND_SEARCH, 1, 0) for b in range(2, 6)],
     "numeric_search_g64": [(2, b, F.KIND_SEARCH, 64, 0) for b in range(2, 10)],
     "hash_g2": [(ph, b, F.KIND_HASH, 2, 0) for ph in (0, 1) for b in range(2, 7)],
@@ -184,6 +190,18 @@ def test_tiled_bins_move_to_the_map_on_reuse(gpu, name):
 def test_forced_plans_identical_bits(gpu, name, variant):
     A, B = cases.build(name)
     got, plan = rvsp.spgemm_device(A, B, choices=PLAN_VARIANTS[variant])
+    assert oracle.digest(got.row_ptr, got.col_idx, got.values) == DIGESTS[name]["digest"]
+
+
+@pytest.mark.parametrize("cap", [512, 1024])
+def test_wave_tiles_on_power_law_digest(gpu, cap):
+    # R-MAT 2^16 (hub rows of thousands of entries, spans of the whole
+    # column range): every long-row numeric bin on wave tiles.
+    name = "cfg3_rmat_s16_seed3"
+    A, B = cases.build(name)
+    choices = [(2, b, F.KIND_BLOCK, 256, cap, 1) for b in range(10, 13)]
+    got, plan = rvsp.spgemm_device(A, B, choices=choices)
+    assert any(p["phase"] == 2 and p["kind"] == "lds_block" and p["rows"] > 0 for p in plan)
     assert oracle.digest(got.row_ptr, got.col_idx, got.values) == DIGESTS[name]["digest"]
 
 

@@ -20,11 +20,16 @@ def test_class_table_layout():
     assert kinds.count(F.KIND_SEARCH) == 7
     assert kinds.count(F.KIND_MAP) == 21
     assert kinds.count(F.KIND_DIRECT) == 7
-    assert kinds.count(F.KIND_BITMAP) == kinds.count(F.KIND_GLOBAL) == kinds.count(F.KIND_BLOCK) == 1
+    assert kinds.count(F.KIND_BITMAP) == kinds.count(F.KIND_GLOBAL) == 1
+    # Long rows: workgroup tiles of 6144, 4096 and 3072 positions, wave
+    # tiles of 512 and 1024.
+    assert sorted((t["variant"], t["cap"]) for t in tab if t["kind"] == F.KIND_BLOCK) == \
+        [(0, 3072), (0, 4096), (0, 6144), (1, 512), (1, 1024)]
+    assert all(t["cap"] == 0 for t in tab if t["kind"] != F.KIND_BLOCK)
     # Every lane count of a sub-wave kind appears once per (kind, table size, variant).
     seen = set()
     for t in tab:
-        key = (t["kind"], t["group"], t["cap_mult"], t["variant"])
+        key = (t["kind"], t["group"], t["cap_mult"], t["variant"], t["cap"])
         assert key not in seen
         seen.add(key)
     assert {t["group"] for t in tab if t["kind"] == F.KIND_HASH} == {1, 2, 4, 8, 16, 32, 64}
