@@ -92,7 +92,7 @@ __device__ __forceinline__ void block_for_each_col(int a_s, int a_e,
                                                    const int* __restrict__ b_rp,
                                                    const int* __restrict__ b_ci,
                                                    int* sh_excl, int* sh_bs, int* etab,
-                                                   int* tmp, F&& f) {
+                                                   int* tmp, F&& f, int len_cap = INT_MAX) {
   const int t = threadIdx.x;
   if (t < kBlockPad) sh_excl[NT + t] = INT_MAX;  // probes past the chunk end
   for (int base = a_s; base < a_e; base += NT) {
@@ -102,6 +102,7 @@ __device__ __forceinline__ void block_for_each_col(int a_s, int a_e,
       const int k = a_ci[ap];
       bs = b_rp[k];
       len = b_rp[k + 1] - bs;
+      if (len >= len_cap) len = 0;  // served elsewhere
     }
     int total;
     const int excl = block_excl_scan<int, NT>(len, tmp, &total);
@@ -137,6 +138,75 @@ __device__ __forceinline__ void block_for_each_col(int a_s, int a_e,
       for (int st = 0; st < kBlockStreams; ++st) f(c[st], p0 + st * NT + t < total);
     }
     __syncthreads();
+  }
+}
+
+// Order-free whole-workgroup product stream (the bitmap kernels: a set
+// union needs no order). Per chunk of NT A entries, entries whose B row has
+// at least kItemMin entries are cut into items of kItem consecutive B
+// positions that the waves take in turn: a wave finds its item's entry by
+// a binary search of the item offsets (one LDS address per step, shared by
+// all lanes) and gathers kItem / 64 columns per lane with no per-product
+// entry lookup. The short entries' products then go through
+// block_for_each_col's flattened stream (the long ones count as empty
+// there). f(col, valid) is called uniformly per wave. Must be reached by
+// every thread of the block; ends with a barrier.
+constexpr int kItemMin = 64;
+constexpr int kItem = 16 * kWave;
+template <int NT, class F>
+__device__ __forceinline__ void block_for_each_col_items(int a_s, int a_e,
+                                                         const int* __restrict__ a_ci,
+                                                         const int* __restrict__ b_rp,
+                                                         const int* __restrict__ b_ci,
+                                                         int* sh_excl, int* sh_bs, int* etab,
+                                                         int* tmp, F&& f) {
+  const int t = threadIdx.x;
+  const int lane = t & (kWave - 1);
+  const int wid = t >> 6;
+  constexpr int kWaves = NT / kWave;
+  bool any_short = false;
+  for (int base = a_s; base < a_e; base += NT) {
+    const int ap = base + t;
+    int bs = 0, len = 0;
+    if (ap < a_e) {
+      const int k = a_ci[ap];
+      bs = b_rp[k];
+      len = b_rp[k + 1] - bs;
+    }
+    const bool lng = len >= kItemMin;
+    any_short |= len > 0 && !lng;
+    const int items = lng ? (len + kItem - 1) / kItem : 0;
+    int total;
+    const int off = block_excl_scan<int, NT>(items, tmp, &total);
+    sh_excl[t] = off;
+    sh_bs[t] = bs;
+    etab[t] = len;
+    __syncthreads();
+    for (int it = wid; it < total; it += kWaves) {
+      // The entry of item `it`: the last e with sh_excl[e] <= it.
+      int lo = 0, n = NT;
+      while (n > 1) {
+        const int half = n >> 1;
+        lo = sh_excl[lo + half] <= it ? lo + half : lo;
+        n -= half;
+      }
+      const int eb = sh_bs[lo], el = etab[lo];
+      const int o0 = (it - sh_excl[lo]) * kItem;
+      int c[kItem / kWave];
+#pragma unroll
+      for (int u = 0; u < kItem / kWave; ++u) {
+        const int o = o0 + u * kWave + lane;
+        c[u] = o < el ? b_ci[eb + o] : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < kItem / kWave; ++u) f(c[u], c[u] >= 0);
+    }
+    __syncthreads();
+  }
+  // Short entries, if the row has any: the flattened stream.
+  if (__syncthreads_or(any_short)) {
+    block_for_each_col<NT>(a_s, a_e, a_ci, b_rp, b_ci, sh_excl, sh_bs, etab, tmp,
+                           std::forward<F>(f), kItemMin);
   }
 }
 
@@ -523,7 +593,7 @@ __global__ __launch_bounds__(NT) void k_bitmap_row(
     }
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x >> 6;
-    block_for_each_col<NT>(A.rp[row], A.rp[row + 1], A.ci, B.rp, B.ci, sh_excl,
+    block_for_each_col_items<NT>(A.rp[row], A.rp[row + 1], A.ci, B.rp, B.ci, sh_excl,
                        sh_bs, sh_etab, tmp, [&](int col, bool valid) {
                          if (valid) {
                            // Bits are only ever set: a bit seen set needs no
@@ -539,7 +609,7 @@ __global__ __launch_bounds__(NT) void k_bitmap_row(
                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
                          }
                        });
-    // block_for_each_col ends with a barrier, which also orders the global
+    // block_for_each_col_items ends with a barrier, which also orders the global
     // atomics above before the reads below (workgroup scope).
     // The span's words are split into one part per wave; a wave reads its
     // part 64 consecutive words at a time (no bank conflicts).
