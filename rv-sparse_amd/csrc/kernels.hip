@@ -32,6 +32,13 @@ namespace rvsp {
 // flattened products are dealt out round-robin (load-balanced search), so a
 // group stays busy whatever the mix of long and short B rows. Order of the
 // calls is unspecified: only for order-free work (count, fill).
+// Keeps the compiler from moving LDS accesses across this point (the
+// hardware already executes one wave's LDS operations in program order), so
+// values written by other lanes of the wave are read from LDS.
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
 template <int G, class F>
 __device__ __forceinline__ void group_for_each_col(int a_s, int a_e,
                                                    const int* __restrict__ a_ci,
@@ -939,12 +946,29 @@ __global__ __launch_bounds__(kBlock) void k_fill_hash(
                         });
   const int c_s = c_rp[row];
   const int n = c_rp[row + 1] - c_s;
-  for (int j = l; j < n; j += G) {
-    const int key = list[j];
-    int rank = 0;
-    for (int q = 0; q < n; ++q) rank += (list[q] < key) ? 1 : 0;
-    c_ci[c_s + rank] = key;
+  // The group's n distinct columns, in hash order, sorted ascending by a
+  // bitonic network over the list padded to a power of two (at most T / 2,
+  // the list's size). A group lies within one wave, whose LDS operations run
+  // in program order, so its compare-exchange steps need no barrier.
+  int P = 1;
+  while (P < n) P <<= 1;
+  for (int j = n + l; j < P; j += G) list[j] = INT_MAX;
+  wave_lds_order();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      for (int i = l; i < (P >> 1); i += G) {
+        const int lo = ((i & ~(jj - 1)) << 1) | (i & (jj - 1));
+        const int hi = lo + jj;
+        const int a = list[lo], b = list[hi];
+        if ((a > b) == ((lo & k) == 0)) {
+          list[lo] = b;
+          list[hi] = a;
+        }
+      }
+      wave_lds_order();
+    }
   }
+  for (int j = l; j < n; j += G) c_ci[c_s + j] = list[j];
 }
 
 template <int G>
@@ -2121,13 +2145,6 @@ constexpr int kWaveHashSpan = 1 << (31 - kWavePosBits);  // hash tiles' column s
 int wave_tile_lds_bytes(int Tw) {
   // vals[Tw] | index (Tw uint2 words, or 2 Tw hash slots) | tags | marks
   return (3 * Tw + kWaveTags + kWave) * 4;
-}
-
-// Keeps the compiler from moving LDS accesses across this point (the
-// hardware already executes one wave's LDS operations in program order), so
-// values written by other lanes of the wave are read from LDS.
-__device__ __forceinline__ void wave_lds_order() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
 template <int G>

@@ -10,7 +10,8 @@ RVSP_SPGEMM_ALGO_HIP_LEARNED descriptors find it) and the samples (.npz).
 
 Training set: R-MAT (a,b,c,d = .57,.19,.19,.05, edge factor 16) at scales
 14-17 with seeds 1..8 and scale 18 with seeds 1..3 (seed 0 is kept for
-evaluation), plus lognormal-row products, uniform-k products, 5-point
+evaluation), plus lognormal-row products (up to 2^22 rows, the size of a
+stacked batch's bins), uniform-k products, 5-point
 Laplacians and banded rows with a few dense rows at three sizes, so that
 every bin of every phase is seen with short and long B rows.
 """
@@ -32,7 +33,7 @@ def training_set():
             yield f"rmat s{scale} seed{seed}", (lambda s=scale, k=seed: (gen.rmat(s, seed=k), None))
     for seed in range(1, 4):
         yield f"rmat s18 seed{seed}", (lambda k=seed: (gen.rmat(18, seed=k), None))
-    for n in (4096, 32768, 131072):
+    for n in (4096, 32768, 131072, 1 << 20, 1 << 22):
         for seed in (1, 2):
             yield f"lognormal n{n} seed{seed}", (
                 lambda n=n, k=seed: (gen.lognormal_rows(n, seed=k),
