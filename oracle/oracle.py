@@ -197,6 +197,97 @@ def merge_f64_errors(errs, tolerance=1e-5):
 _rlib = None
 
 
+REF_MTX_LIB = os.path.join(os.path.dirname(REF_LIB), "libref_mtx.so")
+
+
+def ref_mtx_available():
+    return os.path.exists(REF_MTX_LIB)
+
+
+class _Vec(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("size", C.c_size_t), ("capacity", C.c_size_t),
+                ("elem_size", C.c_size_t)]
+
+
+class _RefCsr(C.Structure):
+    _fields_ = [("row_ptr", C.POINTER(_Vec)), ("col_ind", C.POINTER(_Vec)),
+                ("val", C.POINTER(_Vec))]
+
+
+def ref_read_mtx(path):
+    """The reference's assemble_csr_matrix (tools/src/mtx_to_csr_formatter.c
+    :153-325) on a file: (row_ptr, col_idx, values) arrays, or None when it
+    returns no matrix."""
+    h = C.CDLL(REF_MTX_LIB, mode=C.RTLD_LOCAL)
+    h.assemble_csr_matrix.restype = _RefCsr
+    h.assemble_csr_matrix.argtypes = [C.c_char_p]
+    h.vector_free.argtypes = [C.POINTER(_Vec)]
+    m = h.assemble_csr_matrix(path.encode())
+    if not m.row_ptr:
+        return None
+    out = []
+    for v, dt in ((m.row_ptr, np.int32), (m.col_ind, np.int32), (m.val, np.float32)):
+        n = v.contents.size
+        out.append(np.ctypeslib.as_array((C.c_uint8 * max(n * 4, 1)).from_address(v.contents.data))
+                   [:n * 4].view(dt).copy() if n else np.zeros(0, dt))
+        h.vector_free(v)
+    return tuple(out)
+
+
+def _strtof(text):
+    """Decimal text to FP32 as the C library's strtof (one rounding; a
+    parse through a double would round twice)."""
+    libc = C.CDLL(None)
+    libc.strtof.restype = C.c_float
+    libc.strtof.argtypes = [C.c_char_p, C.c_void_p]
+    return np.float32(libc.strtof(text.encode(), None))
+
+
+def read_mtx(path):
+    """This oracle's restatement of the reference converter in plain
+    Python/numpy (small files): (rows, cols, row_ptr, col_idx, values).
+    Banner pattern/symmetric, 1-based coordinates, the first `L` entries in
+    file order, mirrors of off-diagonal entries appended for symmetric
+    storage, rows in that order, then per row a stable sort by column and
+    repeated columns summed in order in FP32 (mtx_to_csr_formatter.c
+    :74-151, 153-325)."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    words = lines[0].split()
+    pattern, symmetric = words[3] == "pattern", words[4] == "symmetric"
+    k = 1
+    while lines[k].startswith("%"):
+        k += 1
+    M, N, L = (int(x) for x in lines[k].split())
+    toks = " ".join(lines[k + 1:]).split()
+    per = 2 if pattern else 3
+    ent = []
+    for e in range(L):
+        t = toks[e * per:(e + 1) * per]
+        v = np.float32(1.0) if pattern else _strtof(t[2])
+        ent.append((int(t[0]) - 1, int(t[1]) - 1, v))
+    if symmetric:
+        ent += [(c, r, v) for r, c, v in ent if r != c]
+    rows = [[] for _ in range(M)]
+    for r, c, v in ent:
+        rows[r].append((c, v))
+    rp, ci, vals = [0], [], []
+    for row in rows:
+        row = sorted(row, key=lambda cv: cv[0])  # stable
+        i = 0
+        while i < len(row):
+            c, acc = row[i][0], np.float32(row[i][1])
+            i += 1
+            while i < len(row) and row[i][0] == c:
+                acc = np.float32(acc + np.float32(row[i][1]))
+                i += 1
+            ci.append(c)
+            vals.append(acc)
+        rp.append(len(ci))
+    return (M, N, np.array(rp, np.int32), np.array(ci, np.int32),
+            np.array(vals, np.float32))
+
+
 def ref_available():
     return os.path.exists(REF_LIB)
 

@@ -140,6 +140,9 @@ PROTOTYPES = {
     "rvsp_hip_memcpy": (C.c_int, [P, P, SZ, C.c_int]),
     "rvsp_hip_synchronize": (C.c_int, []),
     "rvsp_hip_pool_release": (C.c_int, []),
+    # include/rv_sparse_mtx.h
+    "rvsp_csr_read_mtx": (C.c_int, [C.c_char_p, I32, PM]),
+    "rvsp_csr_canonicalize": (C.c_int, [PM]),
     "rvsp_hip_pool_cached_bytes": (SZ, []),
     "rvsp_hip_device_count": (I32, []),
     "rvsp_csr_check_device": (C.c_int, [I32, I32, I32, P, P, I32P, I32P]),
