@@ -455,6 +455,15 @@ rvsp_status_t estimate(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
   d->count.table = strat.count;
   d->fill.table = strat.fill;
   d->numeric.table = strat.numeric;
+  if (d->algo == RVSP_SPGEMM_ALGO_HIP_TUNED) {
+    // Tuned bin thresholds, keyed by B's mean row length.
+    const int key = rule_len_class(B.rows, B.nnz);
+    PhaseTable* tabs[] = {&d->count.table, &d->fill.table, &d->numeric.table};
+    for (int ph = 0; ph < 3; ++ph) {
+      const int param = tuned_rule(ph, key);
+      if (param > 0) apply_rule(*tabs[ph], ph, param);
+    }
+  }
 
   // Count.
   RVSP_HIP_TRY(launch_classify_count(A, st, d->count.table.rule,
@@ -513,9 +522,48 @@ rvsp_status_t estimate(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
   return RVSP_SUCCESS;
 }
 
+// Re-bins one phase of an estimated descriptor under table t (its rows'
+// sizes are known): classification, partition, offsets, row bases and the
+// global scratch its GLOBAL bins need; t becomes the phase's table.
+rvsp_status_t repartition(rvsp_spgemm_descr* d, int phase, const DevCsr& A, const DevCsr& B,
+                          const PhaseTable& t, hipStream_t s) {
+  const int M = A.rows;
+  const RowStats st = row_stats(d);
+  PhasePlan* plans[] = {&d->count, &d->fill, &d->numeric};
+  PhasePlan& p = *plans[phase];
+  p.table = t;
+  clamp_to_device(p.table);
+  if (phase == PHASE_COUNT)
+    RVSP_HIP_TRY(launch_classify_count(A, st, p.table.rule, (unsigned char*)d->bin_of.p,
+                                       (int*)d->cnt.p, s));
+  else
+    RVSP_HIP_TRY(launch_classify_nnz(A, st, (const int*)d->c_rp.p, p.table.rule,
+                                     (unsigned char*)d->bin_of.p, s,
+                                     phase == PHASE_FILL && d->keep_ok
+                                         ? (const long long*)d->koff.p : nullptr));
+  RVSP_TRY(partition_phase(d, p, M, phase, s));
+  PhasePlan* one[] = {&p};
+  RVSP_TRY(fetch_offsets(d, one, 1, phase, s));
+  if (phase == PHASE_NUMERIC) RVSP_TRY(fetch_row_bases(p, s));
+  RVSP_TRY(ensure_slots(d, p, B.cols, phase == PHASE_NUMERIC, s));
+  return RVSP_SUCCESS;
+}
+
 }  // namespace
 
 namespace rvsp {
+
+rvsp_status_t repartition_phase(rvsp_spgemm_descr* d, int phase, const DevCsr& A,
+                                const DevCsr& B, const PhaseTable& t, hipStream_t s) {
+  return repartition(d, phase, A, B, t, s);
+}
+
+int rule_len_class(int b_rows, int b_nnz) {
+  const double mean = b_rows > 0 ? (double)b_nnz / b_rows : 0.0;
+  int c = 0;
+  while (c < 10 && mean >= (double)(2 << c)) ++c;
+  return c;  // floor(log2(mean)), clamped to [0, 10]
+}
 
 rvsp_status_t run_fill(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
                        int* c_ci, hipStream_t s) {

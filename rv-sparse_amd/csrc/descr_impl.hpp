@@ -177,6 +177,15 @@ rvsp_status_t csr_check_device_key(int rows, int cols, int nnz, const int* rp, c
 rvsp_status_t oneshot_legacy(const rvsp_csr_matrix_t* A, const rvsp_csr_matrix_t* B,
                              rvsp_csr_matrix_t* C, bool i8);
 
+// Re-bins one phase of an estimated descriptor under table t (classification,
+// partition, offsets, GLOBAL scratch); t becomes the phase's table.
+rvsp_status_t repartition_phase(rvsp_spgemm_descr* d, int phase, const DevCsr& A,
+                                const DevCsr& B, const PhaseTable& t, hipStream_t s);
+// Key of the tuned bin thresholds: floor(log2(B's mean row length)) in [0, 10].
+int rule_len_class(int b_rows, int b_nnz);
+// The tuned rule parameter of a phase for that key (apply_rule), or -1.
+int tuned_rule(int phase, int key);
+
 // row_base of the numeric bin whose row list starts at `rows` (-1 if none).
 int numeric_row_base(const rvsp_spgemm_descr* d, const int* rows);
 

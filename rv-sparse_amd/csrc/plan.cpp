@@ -2,6 +2,8 @@
 // override launchable.
 #include "plan.hpp"
 
+#include <algorithm>
+
 #include "common.hpp"
 
 #include <climits>
@@ -152,6 +154,31 @@ Strategy fixed_strategy() {
     t.rule.nbins = b;
   }
   return s;
+}
+
+std::vector<int> rule_params(int phase) {
+  switch (phase) {
+    case PHASE_COUNT: return {4, 2, 3, 5, 6};
+    case PHASE_FILL: return {4, 2, 3};  // the fill list holds half the slots
+    default: return {2048, 1024, 512, 256};
+  }
+}
+
+void apply_rule(PhaseTable& t, int phase, int param) {
+  if (phase == PHASE_NUMERIC) {
+    for (int b = 2; b < t.rule.nbins; ++b)
+      if (t.spec[b].kind == KIND_SEARCH && t.spec[b].cap > param)
+        t.rule.bound[b] = t.rule.bound[b - 1];
+    return;
+  }
+  int last = -1;  // the largest hash bin
+  for (int b = 2; b < t.rule.nbins; ++b)
+    if (t.spec[b].kind == KIND_HASH) last = b;
+  for (int b = 2; b < t.rule.nbins; ++b) {
+    if (t.spec[b].kind != KIND_HASH) continue;
+    if (phase == PHASE_COUNT && b == last) continue;
+    t.rule.bound[b] = std::max(1, (int)((long long)t.spec[b].cap * param / 8));
+  }
 }
 
 bool apply_choice(PhaseTable& t, int phase, int b, const BinChoice& c,

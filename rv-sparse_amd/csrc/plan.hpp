@@ -45,6 +45,19 @@ bool apply_choice(PhaseTable& t, int phase, int b, const BinChoice& c,
 
 const char* kind_name(int kind);
 
+// Bin thresholds (the autotuner's second dimension besides kernel shapes).
+// A phase's rule variant is one integer: COUNT and FILL, the hash bins'
+// admitted load in eighths of their table size (the fixed table's is 4:
+// rows up to half the slots; COUNT keeps its largest hash bin's bound, so
+// the rows reaching the bitmap bins, whose sorted columns the count phase
+// keeps, do not change); NUMERIC, the longest row the searching bins serve
+// (the fixed table's 2048; longer rows go to the long-row bins). The first
+// candidate is the fixed table's.
+std::vector<int> rule_params(int phase);
+// Rewrites the bounds of table t (a fixed-table phase) for param; bins that
+// the variant leaves empty get their predecessor's bound.
+void apply_rule(PhaseTable& t, int phase, int param);
+
 // Largest dynamic LDS per workgroup the current device grants (set once at
 // device initialisation).
 void set_lds_limit(int bytes);

@@ -126,6 +126,13 @@ void ensure_loaded_locked() {
 
 }  // namespace
 
+int tuned_rule(int phase, int key) {
+  std::lock_guard<std::mutex> lock(g_table_mu);
+  ensure_loaded_locked();
+  auto it = g_table.find(TuneKey(phase, 0, key));  // bin 0 (empty rows) holds the rule
+  return it == g_table.end() ? -1 : it->second.choice.cap;
+}
+
 void select_tuned(PhaseTable& t, int phase, const std::vector<BinFeatures>& f) {
   std::lock_guard<std::mutex> lock(g_table_mu);
   ensure_loaded_locked();
@@ -179,6 +186,67 @@ rvsp_status_t autotune_plan(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr&
   RVSP_HIP_TRY(hipEventCreate(&e1));
   PhasePlan* plans[] = {&d->count, &d->fill, &d->numeric};
   rvsp_status_t status = RVSP_SUCCESS;
+  // Bin thresholds first: each rule variant of each phase re-bins the rows
+  // and times all of the phase's bins with the fixed table's shapes; the
+  // fastest variant stays (recorded under bin 0 of the tuning table) and
+  // the per-bin shapes are tuned on its bins below. Count runs first and
+  // numeric last, so the row classification left behind is the numeric
+  // phase's, which the position map below is planned from.
+  {
+    const Strategy fixed = fixed_strategy();
+    const PhaseTable* ftab[] = {&fixed.count, &fixed.fill, &fixed.numeric};
+    const int key = rule_len_class(B.rows, B.nnz);
+    for (int ph = 0; ph < 3 && status == RVSP_SUCCESS; ++ph) {
+      PhasePlan& p = *plans[ph];
+      float best_us = -1.f;
+      int best = 0;
+      for (int param : rule_params(ph)) {
+        PhaseTable t = *ftab[ph];
+        apply_rule(t, ph, param);
+        status = repartition_phase(d, ph, A, B, t, s);
+        if (status != RVSP_SUCCESS) break;
+        auto run_all = [&]() -> hipError_t {
+          for (int b = ph == PHASE_COUNT ? 2 : 1; b < p.table.rule.nbins; ++b) {
+            const int n = p.offsets[b + 1] - p.offsets[b];
+            if (n == 0) continue;
+            const hipError_t e = launch_bin(d, ph, p.table.spec[b],
+                                            (const int*)p.rows.p + p.offsets[b], n, A, B,
+                                            c_ci, c_val, p.table.rule.bound[b], s);
+            if (e != hipSuccess) return e;
+          }
+          return hipSuccess;
+        };
+        hipError_t e = run_all();
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) e = run_all();
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+          status = from_hip(e);
+          break;
+        }
+        const float us = ms * 1000.f / reps;
+        if (getenv("RVSP_TUNE_VERBOSE"))
+          fprintf(stderr, "rvsp tune: phase %d rule %d  %9.2f us\n", ph, param, us);
+        if (best_us < 0.f || us < best_us) {
+          best_us = us;
+          best = param;
+        }
+      }
+      if (status != RVSP_SUCCESS) break;
+      PhaseTable t = *ftab[ph];
+      apply_rule(t, ph, best);
+      status = repartition_phase(d, ph, A, B, t, s);
+      TuneEntry te;
+      te.choice.cap = best;
+      te.us = best_us;
+      std::lock_guard<std::mutex> lock(g_table_mu);
+      ensure_loaded_locked();
+      g_table[TuneKey(ph, 0, key)] = te;
+    }
+  }
   d->map_promoted = true;  // the measured choice stands for later computes
   // Give every searching numeric bin a position map when the memory budget
   // allows, so searching and the map can be timed against each other.
