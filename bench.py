@@ -1036,6 +1036,10 @@ def main():
               "legacy": bench_legacy}
     configs = {}
     for name in extra:
+        # Each workload starts from an empty device buffer cache (its own
+        # untimed first runs fill it), so what one workload left cached does
+        # not decide whether the next one's timed runs allocate.
+        F.check(F.lib().rvsp_hip_pool_release(), "pool_release")
         t0 = time.perf_counter()
         try:
             configs[name] = extras[name](hip, args, F)
