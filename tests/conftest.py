@@ -7,6 +7,17 @@ import pytest
 # kernel can pass by relying on memory the driver happened to clear.
 os.environ.setdefault("RVSP_POOL_POISON", "1")
 
+# One HIP runtime per process: torch brings its own, under the same soname as
+# ROCm's. When a GPU is present, torch touches it before any test module
+# loads the library, so both bind to the same runtime (the GPU tests that
+# build operands with torch need that; the library works on either).
+try:
+    import torch
+    if torch.cuda.is_available():
+        torch.zeros(1, device="cuda")
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (os.path.join(ROOT, "rv-sparse_amd"), ROOT, os.path.dirname(os.path.abspath(__file__))):
     if p not in sys.path:

@@ -216,3 +216,85 @@ def test_batch_learned_and_tuned_algos_keep_bits(gpu):
     for algo in (F.ALGO_HIP_TUNED, F.ALGO_HIP_LEARNED):
         got, _ = spgemm_batch(pairs, algo=algo)
         assert_pairs_match(pairs, got)
+
+
+def test_full_size_2048_pairs_across_int32_groups(gpu):
+    """The full batched workload: 2048 pairs, sizes log-uniform 256-65536,
+    lognormal(ln 8, 1) entries per row capped at 256 (4.2e9 products, so the
+    stack needs at least two int32 groups). Every output word of every pair
+    equals the per-pair descriptor path's (order-sensitive checksums over
+    each array), first compute and graph replays alike, and sampled pairs
+    from both groups equal the oracle bit for bit."""
+    import torch
+    from rvsp import devgen
+    pairs, keep = devgen.lognormal_batch(count=2048, seed=0, device="cuda")
+    n = len(pairs)
+    L.rvsp_hip_pool_release()
+
+    def out_arrays(nnz):
+        tot = int(sum(nnz))
+        ci = torch.full((max(tot, 1),), -1, dtype=torch.int32, device="cuda")
+        val = torch.full((max(tot, 1),), -1, dtype=torch.int32, device="cuda").view(torch.float32)
+        rps = [torch.full((a.rows + 1,), -1, dtype=torch.int32, device="cuda") for a, _ in pairs]
+        st = (F.CsrMatrix * n)()
+        off = 0
+        for p, (A, B) in enumerate(pairs):
+            st[p] = F.CsrMatrix(A.rows, B.cols, int(nnz[p]), rps[p].data_ptr(),
+                                ci.data_ptr() + 4 * off, val.data_ptr() + 4 * off,
+                                F.DTYPE_FP32, F.FORMAT_CSR, 0)
+            off += int(nnz[p])
+        return ci, val, rps, st
+
+    def checksum(t):
+        w = t.reshape(-1).view(torch.int32).long()
+        idx = torch.arange(w.numel(), device=w.device, dtype=torch.long) * 2 + 1
+        return int((w * idx).sum()) % (1 << 64)
+
+    # Per-pair descriptors (FIXED), each pair's arrays checksummed.
+    want = []
+    for A, B in pairs:
+        d = rvsp.Descriptor(algo=F.ALGO_HIP_FIXED, memory=F.MEMORY_DEVICE)
+        nnz = d.estimate(A, B)
+        ci = torch.full((max(nnz, 1),), -1, dtype=torch.int32, device="cuda")
+        val = torch.full((max(nnz, 1),), -1, dtype=torch.int32, device="cuda").view(torch.float32)
+        rp = torch.full((A.rows + 1,), -1, dtype=torch.int32, device="cuda")
+        cm = F.CsrMatrix(A.rows, B.cols, nnz, rp.data_ptr(), ci.data_ptr(), val.data_ptr(),
+                         F.DTYPE_FP32, F.FORMAT_CSR, 0)
+        F.check(d.compute(A, B, cm))
+        want.append((nnz, checksum(rp), checksum(ci[:nnz]), checksum(val[:nnz])))
+        d.close()
+    bt = Batch(algo=F.ALGO_HIP_LEARNED)
+    As, Bs = [a for a, _ in pairs], [b for _, b in pairs]
+    nnz_b = bt.estimate(As, Bs)
+    assert bt.groups() >= 2
+    assert [int(x) for x in nnz_b] == [w[0] for w in want]
+    ci, val, rps, st = out_arrays(nnz_b)
+    offs = np.concatenate([[0], np.cumsum(np.asarray(nnz_b, np.int64))])
+
+    def check_all():
+        for p in range(n):
+            k = int(nnz_b[p])
+            got = (checksum(rps[p]), checksum(ci[offs[p]:offs[p] + k]),
+                   checksum(val[offs[p]:offs[p] + k]))
+            assert got == want[p][1:], f"pair {p}"
+
+    F.check(bt.compute(As, Bs, st))
+    check_all()
+    for _ in range(3):  # position maps, capture, replay
+        F.check(bt.compute(As, Bs, st))
+    assert bt.graph_launches() >= 1
+    val.view(torch.int32).fill_(-1)
+    F.check(bt.compute(As, Bs, st))  # a graph replay rewrites every value
+    check_all()
+    # Oracle on sampled pairs of both groups.
+    for p in (0, 7, n // 2 - 1, n // 2, n - 8, n - 1):
+        A, B = pairs[p][0].to_host(), pairs[p][1].to_host()
+        rp_w, ci_w, v_w = oracle.spgemm(A, B)
+        k = int(nnz_b[p])
+        assert rps[p].cpu().numpy().tobytes() == rp_w.tobytes()
+        assert ci[offs[p]:offs[p] + k].cpu().numpy().tobytes() == ci_w.tobytes()
+        assert val[offs[p]:offs[p] + k].cpu().numpy().tobytes() == v_w.tobytes()
+    bt.close()
+    del ci, val, rps, keep
+    torch.cuda.empty_cache()
+    L.rvsp_hip_pool_release()

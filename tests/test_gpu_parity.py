@@ -311,3 +311,22 @@ def test_raw_oneshot_kernel(gpu):
                                  B.values.ctypes.data, ws.ctypes.data, need.value - 1,
                                  C.byref(rp), C.byref(ci), C.byref(val), C.byref(nnz),
                                  None) == F.ERROR_INVALID_ARGUMENT
+
+
+def test_full_size_banded_with_dense_rows(gpu):
+    # The banded workload at full size: A 500k x 200k (every 100th row 2000
+    # uniform entries over 200k columns), B 200k x 500k: C's dense rows span
+    # most of B's 500k columns (long-row kernels). FIXED and LEARNED plans give
+    # the same bytes; oracle rows cover band rows and dense rows at both ends.
+    A, B = gen.banded_mixed(seed=0)
+    got_f, _ = rvsp.spgemm_device(A, B, algo=F.ALGO_HIP_FIXED)
+    got_l, plan = rvsp.spgemm_device(A, B, algo=F.ALGO_HIP_LEARNED)
+    for k in ("row_ptr", "col_idx", "values"):
+        assert getattr(got_f, k).tobytes() == getattr(got_l, k).tobytes()
+    assert any(p["phase"] == 2 and p["kind"] == "lds_block" and p["rows"] > 0 for p in plan)
+    for r0, r1 in ((0, 3000), (A.rows - 2100, A.rows)):
+        rp, ci, v = oracle.spgemm(A, B, r0, r1)
+        s, e = int(got_f.row_ptr[r0]), int(got_f.row_ptr[r1])
+        assert np.array_equal(got_f.row_ptr[r0:r1 + 1] - s, rp)
+        assert got_f.col_idx[s:e].tobytes() == ci.tobytes()
+        assert got_f.values[s:e].tobytes() == v.tobytes()
