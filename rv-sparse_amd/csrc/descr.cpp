@@ -1035,8 +1035,9 @@ static rvsp_status_t oneshot(const rvsp_csr_matrix_t* A, const rvsp_csr_matrix_t
 // Reference: src/core/spgemm.c:135-211. The FP32 scalar backends and the HIP
 // backend run the HIP descriptor path; the RVV backend selects the RVV
 // strategy, which this build reports as unsupported after the same checks
-// the reference makes first. FP64 and INT8 (legacy kernels in the reference)
-// are not provided.
+// the reference makes first. FP64 x FP64 -> FP64 and INT8 x INT8 -> INT32
+// (the reference's legacy kernels) run on the GPU through legacy.cpp
+// (oneshot_legacy), with each legacy wrapper's status codes.
 rvsp_status_t rvsp_spgemm_csr(const rvsp_csr_matrix_t* A, const rvsp_csr_matrix_t* B,
                               rvsp_csr_matrix_t* C, const rvsp_spgemm_options_t* options) {
   if (!A || !B || !C) return RVSP_ERROR_NULL_POINTER;
