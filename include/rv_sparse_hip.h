@@ -187,6 +187,14 @@ int32_t rvsp_spgemm_batch_groups(rvsp_spgemm_descr_t descr);
 int32_t rvsp_spgemm_batch_graph_launches(rvsp_spgemm_descr_t descr);
 /* on == 0: repeated batch computes launch eagerly instead of from a graph. */
 rvsp_status_t rvsp_spgemm_batch_set_graph(rvsp_spgemm_descr_t descr, int on);
+/* on != 0: the groups' long-row numeric bins (workgroup tiles) run as one
+ * fused launch per tile size after every group's other work, so the long
+ * rows of all groups share one grid; 0 (default) launches them per group
+ * and bin, overlapped with the other groups on the batch's lanes. Output
+ * bits are the same either way. Fusing saves launches on many small groups
+ * but gives up the overlap. Must precede the batch estimation. */
+rvsp_status_t rvsp_spgemm_batch_set_fuse(rvsp_spgemm_descr_t descr, int on);
+
 /* Caps the product count of a group (default and maximum INT32_MAX; 0
  * restores the default). Must precede the batch estimation. */
 rvsp_status_t rvsp_spgemm_batch_set_group_limit(rvsp_spgemm_descr_t descr,

@@ -319,12 +319,63 @@ void build_output_tables(BatchState& bs, const rvsp_csr_matrix_t* C) {
   }
 }
 
+// The groups' long-row numeric bins (workgroup tiles), deferred by their
+// computes: one fused launch per tile size and up to kMaxFusedSegs bins, on
+// the descriptor's stream after every group's fill and other bins.
+rvsp_status_t launch_deferred(BatchState& bs, hipStream_t s) {
+  std::vector<DeferredBin> all;
+  for (BatchGroup& g : bs.groups) {
+    all.insert(all.end(), g.inner->deferred.begin(), g.inner->deferred.end());
+    g.inner->deferred.clear();
+  }
+  if (all.empty()) return RVSP_SUCCESS;
+  const rvsp_spgemm_descr* owner = bs.groups.front().inner;
+  DevCsr A;
+  A.rows = (int)bs.a_row_off.back();
+  A.cols = (int)bs.b_row_off.back();
+  A.nnz = (int)bs.a_nnz_off.back();
+  A.ci = (const int*)bs.a_ci.p;
+  A.val = (const float*)bs.a_val.p;
+  DevCsr B;
+  B.rows = (int)bs.b_row_off.back();
+  B.cols = bs.b_cols;
+  B.nnz = (int)bs.b_nnz_off.back();
+  B.rp = (const int*)bs.b_rp.p;
+  B.ci = (const int*)bs.b_ci.p;
+  B.val = (const float*)bs.b_val.p;
+  std::vector<bool> done(all.size(), false);
+  for (size_t i = 0; i < all.size(); ++i) {
+    if (done[i]) continue;
+    const BinSpec& sp = all[i].spec;
+    TiledSegs segs;
+    int max_row = 0;
+    for (size_t j = i; j < all.size() && segs.n < kMaxFusedSegs; ++j) {
+      if (done[j] || all[j].spec.cap != sp.cap) continue;
+      done[j] = true;
+      TiledSeg& t = segs.seg[segs.n++];
+      t.rows = all[j].rows;
+      t.a_rp = all[j].a_rp;
+      t.c_rp = all[j].c_rp;
+      t.c_ci = all[j].c_ci;
+      t.c_val = all[j].c_val;
+      t.nrows = all[j].nrows;
+      t.start = segs.total;
+      segs.total += all[j].nrows;
+      max_row = std::max(max_row, all[j].max_row);
+    }
+    RVSP_HIP_TRY(launch_numeric_tiled_fused(segs, sp.cap, max_row, A, B, owner->split,
+                                            sp.lds_bytes, s));
+  }
+  return RVSP_SUCCESS;
+}
+
 // The compute body: values into the stack, every group's compute, results
 // out. With `concurrent` the groups alternate between the descriptor's
 // stream and the batch's extra lanes (forked from and joined back into the
 // descriptor's stream); otherwise, as under capture, all run on it.
 rvsp_status_t batch_body(BatchState& bs, const rvsp_csr_matrix_t* C, bool full_out,
                          hipStream_t s, bool concurrent) {
+  for (BatchGroup& g : bs.groups) g.inner->deferred.clear();  // nothing left from a failed call
   RVSP_HIP_TRY(bs.t_vals.launch(s));
   const rvsp_csr_matrix_t Bst = view((int)bs.b_row_off.back(), bs.b_cols, bs.b_nnz_off.back(),
                                      bs.b_rp.p, bs.b_ci.p, bs.b_val.p);
@@ -355,6 +406,7 @@ rvsp_status_t batch_body(BatchState& bs, const rvsp_csr_matrix_t* C, bool full_o
       RVSP_HIP_TRY(hipStreamWaitEvent(s, bs.join_ev[k], 0));
     }
   }
+  RVSP_TRY(launch_deferred(bs, s));
   const SegTable& out = full_out ? bs.t_out_full : bs.t_out_vals;
   if (!out.chunks.empty()) RVSP_HIP_TRY(out.launch(s));
   return RVSP_SUCCESS;
@@ -492,6 +544,7 @@ rvsp_status_t rvsp_spgemm_batch_work_estimation(rvsp_spgemm_descr_t descr, int32
     RVSP_TRY(rvsp_spgemm_set_memory(g.inner, RVSP_MEMORY_DEVICE));
     RVSP_TRY(rvsp_spgemm_set_stream(g.inner, (void*)s));
     RVSP_TRY(rvsp_spgemm_set_async(g.inner, 1));
+    g.inner->defer_block = d->batch_fuse;
     if (!d->choices.empty())
       RVSP_TRY(rvsp_spgemm_set_bin_choices(g.inner, d->choices.data(),
                                            (int32_t)d->choices.size()));
@@ -680,6 +733,13 @@ rvsp_status_t rvsp_spgemm_batch_set_graph(rvsp_spgemm_descr_t descr, int on) {
   if (descr == NULL) return RVSP_ERROR_NULL_POINTER;
   descr->batch_graph = on != 0;
   if (descr->batch) descr->batch->drop_graph();
+  return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_spgemm_batch_set_fuse(rvsp_spgemm_descr_t descr, int on) {
+  if (descr == NULL) return RVSP_ERROR_NULL_POINTER;
+  if (descr->batch && descr->batch->estimated) return RVSP_ERROR_INVALID_ARGUMENT;
+  descr->batch_fuse = on != 0;
   return RVSP_SUCCESS;
 }
 

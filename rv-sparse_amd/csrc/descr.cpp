@@ -643,6 +643,19 @@ rvsp_status_t run_numeric(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B
     if (n == 0) continue;
     const BinSpec& spec = d->numeric.table.spec[b];
     const int* rows = (const int*)d->numeric.rows.p + d->numeric.offsets[b];
+    if (d->defer_block && spec.kind == KIND_BLOCK && spec.variant == 0) {
+      DeferredBin db;
+      db.rows = rows;
+      db.nrows = n;
+      db.max_row = d->numeric.table.rule.bound[b];
+      db.spec = spec;
+      db.a_rp = A.rp;
+      db.c_rp = (const int*)d->c_rp.p;
+      db.c_ci = c_ci;
+      db.c_val = c_val;
+      d->deferred.push_back(db);
+      continue;
+    }
     if (spec.kind == KIND_MAP) {
       RVSP_TRY(ensure_map_built(d, A, B, c_ci, s));
       MapView mv;

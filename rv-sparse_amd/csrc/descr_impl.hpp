@@ -16,6 +16,19 @@ namespace rvsp {
 
 struct BatchState;
 
+// A long-row numeric bin whose launch the caller (a batch) fuses with other
+// descriptors' (defer_block): its rows and the arrays the kernel needs.
+struct DeferredBin {
+  const int* rows = nullptr;
+  int nrows = 0;
+  int max_row = 0;
+  BinSpec spec;
+  const int* a_rp = nullptr;
+  const int* c_rp = nullptr;
+  const int* c_ci = nullptr;
+  float* c_val = nullptr;
+};
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -115,6 +128,7 @@ struct rvsp_spgemm_descr {
   // descriptor per group of pairs; settings apply at the next estimation.
   rvsp::BatchState* batch = nullptr;
   bool batch_graph = true;
+  bool batch_fuse = false;    // fuse the groups' long-row numeric launches
   long long batch_limit = 0;  // products per group; 0 = INT32_MAX
 
   // Set on a batch's inner descriptors whose groups run on two streams at
@@ -122,6 +136,12 @@ struct rvsp_spgemm_descr {
   // it waits for a whole CU to drain of the other stream's workgroups) run
   // their bitmaps in global scratch instead and share CUs.
   bool shared_cu = false;
+
+  // Set on a batch's inner descriptors: numeric bins on workgroup tiles
+  // (KIND_BLOCK variant 0) are not launched by run_numeric but appended to
+  // `deferred`, and the batch launches all groups' such bins at once.
+  bool defer_block = false;
+  std::vector<rvsp::DeferredBin> deferred;
 
   // Set by the one-shot path: work_estimation checks the staged operands
   // for canonical CSR on the device before any other kernel reads them.

@@ -1837,31 +1837,35 @@ __device__ __forceinline__ int tiled_index_pos(const uint2* idx, int clo, int co
   return (int)v.y + __popc(v.x & ((1u << (d & 31)) - 1u));
 }
 
-__global__ __launch_bounds__(kBlock) void k_numeric_tiled(
-    const int* __restrict__ rows, int nrows, int T, DevCsr A, DevCsr B,
-    const int* __restrict__ c_rp, const int* __restrict__ c_ci,
-    float* __restrict__ c_val, SplitTable split) {
-  extern __shared__ __attribute__((aligned(16))) int tiled_lds_raw[];
-  __shared__ int scan_tmp[kBlock / kWave];
-  const TiledLds L = tiled_lds(tiled_lds_raw, T);
+// The LDS set-up every tiled workgroup does once, before its first row.
+__device__ __forceinline__ void tiled_init(const TiledLds& L) {
+  const int tid = threadIdx.x;
+  if (tid < kPrePad) L.pre[kBlock + tid] = INT_MAX;  // probes past the chunk end
+  for (int i = tid; i < kTiledWaves * kTagSlots; i += kBlock) L.tags[i] = INT_MAX;
+  __syncthreads();
+}
+
+// One workgroup's part of a long row: run blockIdx.y of the row's runs
+// (block-uniform arguments; every thread of the block must call it).
+__device__ __forceinline__ void tiled_row(const TiledLds& L, int* scan_tmp, int row, int T,
+                                          const DevCsr& A, const DevCsr& B,
+                                          const int* __restrict__ c_rp,
+                                          const int* __restrict__ c_ci,
+                                          float* __restrict__ c_val, const SplitTable& split) {
   const int span = 32 * tiled_index_words(T);  // columns a tile may span
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wid = tid >> 6;
   const unsigned long long below = (1ull << lane) - 1ull;
   int* my_tags = L.tags + wid * kTagSlots;
-  if (tid < kPrePad) L.pre[kBlock + tid] = INT_MAX;  // probes past the chunk end
-  for (int i = tid; i < kTiledWaves * kTagSlots; i += kBlock) L.tags[i] = INT_MAX;
-  __syncthreads();
-  for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
-    const int row = rows[r];
+  {
     const int c_s = c_rp[row];
     const int n = c_rp[row + 1] - c_s;
     // This workgroup's run: one of up to gridDim.y equal parts of the row.
     // (32-bit arithmetic only: 64-bit division expands into fused float
     // operations, which no device code here may contain.)
     const int runs = min((int)gridDim.y, n / T + (n % T != 0));
-    if ((int)blockIdx.y >= runs) continue;
+    if ((int)blockIdx.y >= runs) return;
     const int rq = n / runs, rr = n % runs, y = blockIdx.y;
     const int P0 = rq * y + rr * y / runs;                 // floor(n * y / runs)
     const int P1 = rq * (y + 1) + rr * (y + 1) / runs;
@@ -2113,6 +2117,40 @@ __global__ __launch_bounds__(kBlock) void k_numeric_tiled(
       p0 = pe;
       first = false;
     }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_numeric_tiled(
+    const int* __restrict__ rows, int nrows, int T, DevCsr A, DevCsr B,
+    const int* __restrict__ c_rp, const int* __restrict__ c_ci,
+    float* __restrict__ c_val, SplitTable split) {
+  extern __shared__ __attribute__((aligned(16))) int tiled_lds_raw[];
+  __shared__ int scan_tmp[kBlock / kWave];
+  const TiledLds L = tiled_lds(tiled_lds_raw, T);
+  tiled_init(L);
+  for (int r = blockIdx.x; r < nrows; r += gridDim.x)
+    tiled_row(L, scan_tmp, rows[r], T, A, B, c_rp, c_ci, c_val, split);
+}
+
+// The tiled kernel over the long-row bins of several descriptors at once
+// (the groups of a batch): segment i covers workgroup rows [start, start +
+// nrows) of the fused grid, with its own row list, A row pointers (a row
+// range of the stacked A) and C arrays; A's columns and values and all of
+// B are shared. One launch replaces one per bin and group, so the groups'
+// long rows share the GPU without a tail per launch.
+__global__ __launch_bounds__(kBlock) void k_numeric_tiled_fused(TiledSegs segs, int T, DevCsr A,
+                                                                DevCsr B, SplitTable split) {
+  extern __shared__ __attribute__((aligned(16))) int tiled_lds_raw[];
+  __shared__ int scan_tmp[kBlock / kWave];
+  const TiledLds L = tiled_lds(tiled_lds_raw, T);
+  tiled_init(L);
+  for (int x = blockIdx.x; x < segs.total; x += gridDim.x) {
+    int i = 0;
+    while (i + 1 < segs.n && segs.seg[i + 1].start <= x) ++i;  // uniform
+    const TiledSeg& sg = segs.seg[i];
+    DevCsr Ai = A;
+    Ai.rp = sg.a_rp;
+    tiled_row(L, scan_tmp, sg.rows[x - sg.start], T, Ai, B, sg.c_rp, sg.c_ci, sg.c_val, split);
   }
 }
 
@@ -2394,6 +2432,18 @@ static hipError_t launch_numeric_g(const BinSpec& b, const int* rows, int nrows,
   return hipGetLastError();
 }
 
+hipError_t launch_numeric_tiled_fused(const TiledSegs& segs, int T, int max_row, const DevCsr& A,
+                                      const DevCsr& B, const SplitTable& split, int lds_bytes,
+                                      hipStream_t s) {
+  if (segs.n <= 0 || segs.total <= 0) return hipSuccess;
+  const long long most = max_row > 0 ? max_row : 1;
+  const int ytiles = (int)std::min<long long>(16, (most + T - 1) / T);
+  const int gx = (std::min(segs.total, 1 << 20) + 7) / 8 * 8;
+  hipLaunchKernelGGL(k_numeric_tiled_fused, dim3(gx, ytiles), dim3(kBlock), lds_bytes, s, segs,
+                     T, A, B, split);
+  return hipGetLastError();
+}
+
 hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
                               const DevCsr& A, const DevCsr& B, RowStats st,
                               const int* c_rp, const int* c_ci, float* c_val,
@@ -2556,6 +2606,7 @@ hipError_t init_kernel_attributes(int lds_bytes) {
   RVSP_ALLOW((k_bitmap_row<false, kBitmapFill, kWideBlock>))
   RVSP_ALLOW((k_bitmap_row<false, kBitmapCountKeep, kWideBlock>))
   RVSP_ALLOW(k_numeric_rowblock<false>) RVSP_ALLOW(k_numeric_tiled) RVSP_ALLOW(k_numeric_wave)
+  RVSP_ALLOW(k_numeric_tiled_fused)
 #define RVSP_ALLOW_MAP(G)                                                     \
   RVSP_ALLOW((k_numeric_map<G, unsigned char, 0>))                             \
   RVSP_ALLOW((k_numeric_map<G, unsigned char, 1>))                             \

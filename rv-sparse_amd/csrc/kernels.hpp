@@ -197,6 +197,30 @@ hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
 
 // Dynamic LDS of the tiled long-row numeric kernel for tiles of T positions.
 int tiled_lds_bytes(int T);
+
+// Long-row bins of several descriptors in one tiled launch (a batch's
+// groups): per segment its row list and count, the first fused workgroup
+// row it owns, its A row pointers and its C arrays. Passed by value.
+struct TiledSeg {
+  const int* rows;
+  const int* a_rp;
+  const int* c_rp;
+  const int* c_ci;
+  float* c_val;
+  int nrows;
+  int start;
+};
+constexpr int kMaxFusedSegs = 32;
+struct TiledSegs {
+  int n = 0;
+  int total = 0;
+  TiledSeg seg[kMaxFusedSegs];
+};
+// One launch over every segment (tiles of T positions, max_row the longest
+// row any segment's bin admits); A's columns and values and B are shared.
+hipError_t launch_numeric_tiled_fused(const TiledSegs& segs, int T, int max_row, const DevCsr& A,
+                                      const DevCsr& B, const SplitTable& split, int lds_bytes,
+                                      hipStream_t s);
 // Default tile of the long-row numeric kernel: three workgroups fit a CU.
 constexpr int kTiledPositions = 6144;
 // Dynamic LDS of the wave-tile long-row numeric kernel (KIND_BLOCK variant

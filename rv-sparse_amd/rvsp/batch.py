@@ -12,7 +12,8 @@ from .api import DeviceCsr, device_alloc, device_free, from_device
 class Batch:
     """A descriptor used for batched products C[p] = A[p] * B[p]."""
 
-    def __init__(self, algo=F.ALGO_DEFAULT, group_limit=None, graph=True, choices=None):
+    def __init__(self, algo=F.ALGO_DEFAULT, group_limit=None, graph=True, choices=None,
+                 fuse=False):
         self.handle = C.c_void_p()
         check(lib().rvsp_spgemm_descr_create(C.byref(self.handle)), "descr_create")
         check(lib().rvsp_spgemm_set_algo(self.handle, algo), "set_algo")
@@ -25,6 +26,7 @@ class Batch:
             check(lib().rvsp_spgemm_batch_set_group_limit(self.handle, group_limit),
                   "batch_set_group_limit")
         check(lib().rvsp_spgemm_batch_set_graph(self.handle, 1 if graph else 0), "set_graph")
+        check(lib().rvsp_spgemm_batch_set_fuse(self.handle, 1 if fuse else 0), "set_fuse")
         self.c_nnz = None
 
     @staticmethod
@@ -128,10 +130,10 @@ def upload_pairs(pairs):
 
 
 def spgemm_batch(pairs, algo=F.ALGO_DEFAULT, contiguous=True, group_limit=None, graph=True,
-                 repeats=1, choices=None):
+                 repeats=1, choices=None, fuse=False):
     """Host pairs in, host products out, through the batched device path."""
     As, Bs, owned = upload_pairs(pairs)
-    bt = Batch(algo=algo, group_limit=group_limit, graph=graph, choices=choices)
+    bt = Batch(algo=algo, group_limit=group_limit, graph=graph, choices=choices, fuse=fuse)
     out = None
     try:
         nnz = bt.estimate(As, Bs)

@@ -68,11 +68,15 @@ def test_batch_of_row_blocks_shares_b(gpu):
         assert g.values.tobytes() == v[s:e].tobytes()
 
 
-def test_row_blocks_in_groups_share_the_split_table(gpu):
+@pytest.mark.parametrize("fuse", [True, False])
+@pytest.mark.parametrize("repeats", [1, 4])
+def test_row_blocks_in_groups_share_the_split_table(gpu, fuse, repeats):
     """Row blocks of C = A*A in several groups, every long row through the
     tiled kernel with small tiles (many tiles per row, narrowed B rows): the
     groups share one split table of the stacked B and every block still
-    equals the matching rows of the product."""
+    equals the matching rows of the product, whether the groups' long-row
+    bins run as one fused launch (several bins and tile sizes across the
+    groups, graph replays included) or one launch per bin and group."""
     A = gen.rmat(12, seed=5)
     cuts = [0, 300, 1100, 2600, A.rows]
     blocks = []
@@ -82,8 +86,9 @@ def test_row_blocks_in_groups_share_the_split_table(gpu):
         blocks.append(Csr(r1 - r0, A.cols, rp, A.col_idx[s:e], A.values[s:e]))
     pairs = [(blk, A) for blk in blocks]
     ops = [int(np.diff(A.row_ptr)[blk.col_idx].sum()) for blk in blocks]
-    choices = [(2, b, F.KIND_BLOCK, 256, 256) for b in range(2, 13)]
-    got, groups = spgemm_batch(pairs, group_limit=max(ops), choices=choices)
+    choices = [(2, b, F.KIND_BLOCK, 256, 256 if b % 2 else 512) for b in range(2, 13)]
+    got, groups = spgemm_batch(pairs, group_limit=max(ops), choices=choices, fuse=fuse,
+                               repeats=repeats)
     assert groups >= 3
     rp, ci, v = oracle.spgemm(A, A)
     for (r0, r1), g in zip(zip(cuts[:-1], cuts[1:]), got):
