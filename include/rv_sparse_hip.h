@@ -194,6 +194,18 @@ rvsp_status_t rvsp_spgemm_batch_set_graph(rvsp_spgemm_descr_t descr, int on);
  * bits are the same either way. Fusing saves launches on many small groups
  * but gives up the overlap. Must precede the batch estimation. */
 rvsp_status_t rvsp_spgemm_batch_set_fuse(rvsp_spgemm_descr_t descr, int on);
+/* on != 0 (default): from the first numeric-only batch compute on, every
+ * group's numeric bin of rows up to 2048 entries long (and of rows fed by a
+ * single B row) runs as one persistent launch over a product-balanced work
+ * queue of (group, bin, row block) items, captured with the rest of the
+ * compute in the batch's graph; the position map it reads (16 bits per
+ * product) is built by the first numeric-only compute. 0: one launch per bin
+ * and group. Output bits are the same either way. No reference counterpart
+ * (batching is not part of the reference). */
+rvsp_status_t rvsp_spgemm_batch_set_persistent(rvsp_spgemm_descr_t descr, int on);
+/* Segments (group, bin) served by the persistent launch; 0 before the
+ * first numeric-only compute or when it is off. */
+int32_t rvsp_spgemm_batch_stream_segments(rvsp_spgemm_descr_t descr);
 
 /* Caps the product count of a group (default and maximum INT32_MAX; 0
  * restores the default). Must precede the batch estimation. */

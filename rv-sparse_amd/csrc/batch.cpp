@@ -17,6 +17,8 @@
 // When the caller's C arrays of a group sit back to back in memory, the group
 // writes them in place; otherwise it writes its own buffer and one segmented
 // copy moves every pair's part out.
+#include <stddef.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -111,6 +113,17 @@ struct BatchState {
   hipGraphExec_t exec = nullptr;
   std::vector<const void*> graph_key;
   bool graph_failed = false;
+  // Persistent row-stream numeric pass over every group's short-row numeric
+  // bins (numeric-only computes): per segment its list-order product offsets,
+  // queue-item starts and map; the segment table and the launch's queue.
+  struct StreamPart {
+    DevBuf soff, items, map;
+  };
+  std::vector<StreamPart> parts;
+  std::vector<StreamSeg> segs;
+  DevBuf d_segs, d_queue, s_weights, s_tiles, s_totals;
+  int stream_items = 0, stream_capw = 0, stream_grid = 0;
+  bool stream_ready = false, stream_failed = false;
   int graph_launches = 0;
   int repeats = 0;  // consecutive numeric-only computes with the same arrays
   // Eager computes run the groups on kLanes streams at once (fork and join
@@ -167,8 +180,28 @@ struct BatchState {
     }
     groups.clear();
   }
+  void drop_stream() {
+    for (StreamPart& sp : parts) {
+      release(sp.soff);
+      release(sp.items);
+      release(sp.map);
+    }
+    parts.clear();
+    segs.clear();
+    for (BatchGroup& g : groups)
+      if (g.inner) g.inner->stream_bins = 0;
+    stream_ready = false;
+    stream_failed = false;
+    stream_items = 0;
+  }
   ~BatchState() {
     drop_graph();
+    drop_stream();
+    release(d_segs);
+    release(d_queue);
+    release(s_weights);
+    release(s_tiles);
+    release(s_totals);
     drop_groups();
     drop_lanes();
     DevBuf* bufs[] = {&a_rp, &a_ci, &a_val, &b_rp, &b_ci, &b_val, &ops};
@@ -192,6 +225,7 @@ void destroy_batch(rvsp_spgemm_descr* d) {
 void invalidate_batch(rvsp_spgemm_descr* d) {
   if (!d->batch) return;
   d->batch->drop_graph();
+  d->batch->drop_stream();
   d->batch->out_filled = false;
   d->batch->last_key.clear();
   for (BatchGroup& g : d->batch->groups) rvsp_spgemm_invalidate_structure(g.inner);
@@ -319,6 +353,167 @@ void build_output_tables(BatchState& bs, const rvsp_csr_matrix_t* C) {
   }
 }
 
+// Sets up the persistent row-stream numeric pass after the first full
+// compute: every group's numeric bin of rows fed by one B row (identity
+// positions) and every bin of rows up to kStreamMaxRow long becomes a
+// segment; per segment the list-order product offsets, the queue-item
+// starts, and for non-identity bins the position map (built on the device
+// from C's columns, then its order bits). Synchronous. On success the
+// groups skip those bins; on failure (memory, too many segments) nothing
+// changes and the per-bin launches stay.
+rvsp_status_t build_stream(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream_t s) {
+  bs.drop_stream();
+  bs.stream_failed = true;  // until every step below has succeeded
+  struct Pick {
+    int group, bin, nrows, row_base;
+    const int* rows;
+    bool ident;
+  };
+  std::vector<Pick> picks;
+  int max_rows = 0;
+  for (size_t gi = 0; gi < bs.groups.size(); ++gi) {
+    const rvsp_spgemm_descr* in = bs.groups[gi].inner;
+    const PhasePlan& p = in->numeric;
+    for (int b = 1; b < p.table.rule.nbins; ++b) {
+      const int n = bin_rows(p, b);
+      if (n == 0) continue;
+      const bool ident = b == 1;
+      // Rows of bin b are at most bound[b] long, except in the last bin,
+      // which takes every row no other bin admits.
+      if (!ident && (b == p.table.rule.nbins - 1 || p.table.rule.bound[b] > kStreamMaxRow))
+        continue;
+      if ((int)picks.size() == kStreamMaxSegs) break;
+      Pick pk;
+      pk.group = (int)gi;
+      pk.bin = b;
+      pk.nrows = n;
+      pk.rows = (const int*)p.rows.p + p.offsets[b];
+      pk.row_base = (size_t)b < p.row_base.size() ? p.row_base[b] : -1;
+      pk.ident = ident;
+      picks.push_back(pk);
+      max_rows = std::max(max_rows, n);
+    }
+  }
+  if (picks.empty()) return RVSP_SUCCESS;
+  const int ns = (int)picks.size();
+  RVSP_TRY(ensure(bs.s_weights, (size_t)max_rows * 8));
+  RVSP_TRY(ensure(bs.s_tiles, scan_tile_count(max_rows) * 8));
+  RVSP_TRY(ensure(bs.s_totals, (size_t)ns * 8));
+  bs.parts.resize(ns);
+  for (int j = 0; j < ns; ++j) {
+    const Pick& pk = picks[j];
+    RVSP_TRY(ensure(bs.parts[j].soff, ((size_t)pk.nrows + 1) * 8));
+    RVSP_HIP_TRY(launch_stream_weights(pk.row_base >= 0 ? nullptr : pk.rows, pk.row_base,
+                                       pk.nrows, (const long long*)bs.groups[pk.group].inner->ops.p,
+                                       (long long*)bs.s_weights.p, s));
+    RVSP_HIP_TRY(launch_scan64((const long long*)bs.s_weights.p, pk.nrows,
+                               (long long*)bs.parts[j].soff.p, (long long*)bs.s_tiles.p,
+                               (long long*)bs.s_totals.p + j, s));
+  }
+  std::vector<long long> totals(ns, 0);
+  RVSP_HIP_TRY(hipMemcpyAsync(totals.data(), bs.s_totals.p, (size_t)ns * 8,
+                              hipMemcpyDeviceToHost, s));
+  RVSP_HIP_TRY(hipStreamSynchronize(s));
+  size_t map_bytes = 0;
+  long long items = 0;
+  int capw = 0;
+  for (int j = 0; j < ns; ++j) {
+    if (!picks[j].ident) {
+      map_bytes += (size_t)totals[j] * 2;
+      capw = std::max(capw, bs.groups[picks[j].group].inner->numeric.table.rule.bound[picks[j].bin]);
+    }
+    items += (totals[j] + kStreamItem - 1) / kStreamItem;
+  }
+  capw = std::min(capw, kStreamMaxRow);
+  const size_t budget = std::min<size_t>((size_t)32 << 30, device_free_bytes() / 3);
+  if (map_bytes > budget || items > INT_MAX / 2) return RVSP_SUCCESS;
+  DevCsr A, B;
+  A.rows = (int)bs.a_row_off.back();
+  A.cols = (int)bs.b_row_off.back();
+  A.nnz = (int)bs.a_nnz_off.back();
+  A.ci = (const int*)bs.a_ci.p;
+  A.val = (const float*)bs.a_val.p;
+  B.rows = (int)bs.b_row_off.back();
+  B.cols = bs.b_cols;
+  B.nnz = (int)bs.b_nnz_off.back();
+  B.rp = (const int*)bs.b_rp.p;
+  B.ci = (const int*)bs.b_ci.p;
+  B.val = (const float*)bs.b_val.p;
+  bs.segs.assign(ns, StreamSeg());
+  int item0 = 0;
+  for (int j = 0; j < ns; ++j) {
+    const Pick& pk = picks[j];
+    const BatchGroup& g = bs.groups[pk.group];
+    BatchState::StreamPart& part = bs.parts[j];
+    const int nitems = (int)((totals[j] + kStreamItem - 1) / kStreamItem);
+    RVSP_TRY(ensure(part.items, ((size_t)nitems + 1) * 4));
+    RVSP_HIP_TRY(launch_stream_items((const long long*)part.soff.p, pk.nrows, nitems,
+                                     (int*)part.items.p, s));
+    if (!pk.ident && totals[j] > 0) RVSP_TRY(ensure(part.map, (size_t)totals[j] * 2));
+    StreamSeg& sg = bs.segs[j];
+    sg.rows = pk.row_base >= 0 ? nullptr : pk.rows;
+    sg.row_base = pk.row_base >= 0 ? pk.row_base : 0;
+    sg.nrows = pk.nrows;
+    sg.a_rp = (const int*)bs.a_rp.p + g.r0;
+    sg.c_rp = (const int*)g.inner->c_rp.p;
+    sg.c_ci = g.in_place ? (const int*)C[g.p0].col_idx : (const int*)g.c_ci.p;
+    sg.c_val = g.in_place ? (float*)C[g.p0].values : (float*)g.c_val.p;
+    sg.soff = (const long long*)part.soff.p;
+    sg.item_first = (const int*)part.items.p;
+    sg.map = pk.ident ? nullptr : (unsigned short*)part.map.p;
+    sg.total = totals[j];
+    sg.c_nnz = g.c_nnz;
+    sg.item0 = item0;
+    item0 += nitems;
+  }
+  RVSP_TRY(ensure(bs.d_segs, (size_t)ns * sizeof(StreamSeg)));
+  RVSP_HIP_TRY(hipMemcpyAsync(bs.d_segs.p, bs.segs.data(), (size_t)ns * sizeof(StreamSeg),
+                              hipMemcpyHostToDevice, s));
+  if (!bs.d_queue.p) RVSP_TRY(ensure(bs.d_queue, sizeof(StreamQueue)));
+  RVSP_HIP_TRY(hipMemsetAsync(bs.d_queue.p, 0, sizeof(StreamQueue), s));
+  // Map entries (positions and row marks), then their order bits.
+  const int bgrid = stream_grid(true, capw, item0);
+  RVSP_HIP_TRY(launch_stream(true, (const StreamSeg*)bs.d_segs.p, ns, item0, A, B,
+                             (StreamQueue*)bs.d_queue.p, capw, bgrid, s));
+  for (int j = 0; j < ns; ++j)
+    if (!picks[j].ident) RVSP_HIP_TRY(launch_stream_rounds(bs.segs[j].map, totals[j], s));
+  unsigned bad = 0;
+  RVSP_HIP_TRY(hipMemcpyAsync(&bad, (char*)bs.d_queue.p + offsetof(StreamQueue, bad),
+                              sizeof(bad), hipMemcpyDeviceToHost, s));
+  RVSP_HIP_TRY(hipStreamSynchronize(s));
+  if (bad) return RVSP_ERROR_DEVICE;  // a map that does not match the structure
+  bs.stream_items = item0;
+  bs.stream_capw = capw;
+  bs.stream_grid = stream_grid(false, capw, item0);
+  for (const Pick& pk : picks) bs.groups[pk.group].inner->stream_bins |= 1u << pk.bin;
+  bs.stream_failed = false;
+  bs.stream_ready = true;
+  return RVSP_SUCCESS;
+}
+
+// RVSP_STREAM_VERIFY=1: batch computes synchronise and report writes the
+// persistent pass refused as RVSP_ERROR_DEVICE.
+bool stream_verify() {
+  static const bool on = [] {
+    const char* e = getenv("RVSP_STREAM_VERIFY");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+
+rvsp_status_t launch_stream_numeric(BatchState& bs, hipStream_t s) {
+  DevCsr A, B;
+  A.ci = (const int*)bs.a_ci.p;
+  A.val = (const float*)bs.a_val.p;
+  B.rp = (const int*)bs.b_rp.p;
+  B.ci = (const int*)bs.b_ci.p;
+  B.val = (const float*)bs.b_val.p;
+  RVSP_HIP_TRY(launch_stream(false, (const StreamSeg*)bs.d_segs.p, (int)bs.segs.size(),
+                             bs.stream_items, A, B, (StreamQueue*)bs.d_queue.p, bs.stream_capw,
+                             bs.stream_grid, s));
+  return RVSP_SUCCESS;
+}
+
 // The groups' long-row numeric bins (workgroup tiles), deferred by their
 // computes: one fused launch per tile size and up to kMaxFusedSegs bins, on
 // the descriptor's stream after every group's fill and other bins.
@@ -385,6 +580,9 @@ rvsp_status_t batch_body(BatchState& bs, const rvsp_csr_matrix_t* C, bool full_o
     for (int k = 0; k < BatchState::kLanes - 1; ++k)
       RVSP_HIP_TRY(hipStreamWaitEvent(bs.lane[k], bs.fork_ev, 0));
   }
+  // Numeric-only computes: every group's streamed bins in one launch (the
+  // groups' own computes skip them).
+  if (bs.stream_ready && !full_out) RVSP_TRY(launch_stream_numeric(bs, s));
   int gi = 0;
   for (BatchGroup& g : bs.groups) {
     const int lane = conc ? gi % BatchState::kLanes : 0;
@@ -636,6 +834,7 @@ rvsp_status_t rvsp_spgemm_batch_compute(rvsp_spgemm_descr_t descr, int32_t count
   std::vector<const void*> sk = struct_key(count, A, B);
   if (sk != bs.struct_key) {
     bs.drop_graph();
+    bs.drop_stream();
     build_struct_table(bs, A, B);
     RVSP_TRY(bs.t_struct.upload(s));
     RVSP_HIP_TRY(bs.t_struct.launch(s));
@@ -653,6 +852,7 @@ rvsp_status_t rvsp_spgemm_batch_compute(rvsp_spgemm_descr_t descr, int32_t count
   std::vector<const void*> ok = output_key(count, C);
   if (ok != bs.out_key) {
     bs.drop_graph();
+    bs.drop_stream();
     for (BatchGroup& g : bs.groups) {
       bool in_place = true;
       for (int p = g.p0 + 1; p < g.p1 && in_place; ++p) {
@@ -679,6 +879,16 @@ rvsp_status_t rvsp_spgemm_batch_compute(rvsp_spgemm_descr_t descr, int32_t count
   const bool repeat = bs.out_filled && key == bs.last_key;
   bs.last_key = key;
   bs.repeats = repeat ? bs.repeats + 1 : 0;
+  // The first numeric-only compute sets up the persistent row-stream pass
+  // (structure and output arrays are those of the last full compute).
+  if (repeat && d->batch_persistent && !bs.stream_ready && !bs.stream_failed) {
+    const rvsp_status_t st = build_stream(bs, C, s);
+    if (st != RVSP_SUCCESS) {
+      bs.drop_stream();
+      bs.stream_failed = true;
+      (void)hipGetLastError();
+    }
+  }
   // The first numeric-only compute runs eagerly (the groups may switch to
   // their position maps, which synchronises); the second is captured.
   if (repeat && bs.repeats >= 2 && d->batch_graph && !bs.graph_failed) {
@@ -712,6 +922,15 @@ rvsp_status_t rvsp_spgemm_batch_compute(rvsp_spgemm_descr_t descr, int32_t count
     bs.out_filled = true;
   }
   if (!d->async) RVSP_HIP_TRY(hipStreamSynchronize(s));
+  if (bs.stream_ready && stream_verify()) {
+    // Diagnostic mode: every compute checks that the persistent pass
+    // refused no write.
+    unsigned bad = 0;
+    RVSP_HIP_TRY(hipMemcpyAsync(&bad, (char*)bs.d_queue.p + offsetof(StreamQueue, bad),
+                                sizeof(bad), hipMemcpyDeviceToHost, s));
+    RVSP_HIP_TRY(hipStreamSynchronize(s));
+    if (bad) return RVSP_ERROR_DEVICE;
+  }
   for (int p = 0; p < count; ++p) {
     C[p].nnz = bs.c_nnz[p];
     C[p].format = RVSP_FORMAT_CSR;
@@ -734,6 +953,21 @@ rvsp_status_t rvsp_spgemm_batch_set_graph(rvsp_spgemm_descr_t descr, int on) {
   descr->batch_graph = on != 0;
   if (descr->batch) descr->batch->drop_graph();
   return RVSP_SUCCESS;
+}
+
+rvsp_status_t rvsp_spgemm_batch_set_persistent(rvsp_spgemm_descr_t descr, int on) {
+  if (descr == NULL) return RVSP_ERROR_NULL_POINTER;
+  descr->batch_persistent = on != 0;
+  if (descr->batch) {
+    descr->batch->drop_graph();
+    descr->batch->drop_stream();
+  }
+  return RVSP_SUCCESS;
+}
+
+int32_t rvsp_spgemm_batch_stream_segments(rvsp_spgemm_descr_t descr) {
+  if (descr == NULL || !descr->batch || !descr->batch->stream_ready) return 0;
+  return (int32_t)descr->batch->segs.size();
 }
 
 rvsp_status_t rvsp_spgemm_batch_set_fuse(rvsp_spgemm_descr_t descr, int on) {

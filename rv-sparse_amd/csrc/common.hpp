@@ -70,6 +70,25 @@ __device__ __forceinline__ T group_incl_scan(T v) {
   return v;
 }
 
+// Inclusive running maximum over the G consecutive lanes of a group.
+template <int G>
+__device__ __forceinline__ int group_incl_max(int v) {
+  const int l = threadIdx.x & (G - 1);
+#pragma unroll
+  for (int d = 1; d < G; d <<= 1) {
+    const int t = __shfl_up(v, d, G);
+    if (l >= d) v = max(v, t);
+  }
+  return v;
+}
+
+// Keeps the compiler from moving LDS accesses across this point (the
+// hardware already executes one wave's LDS operations in program order), so
+// values written by other lanes of the wave are read from LDS.
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
 template <int G, typename T>
 __device__ __forceinline__ T group_sum(T v) {
 #pragma unroll

@@ -303,7 +303,6 @@ rvsp_status_t fetch_offsets(rvsp_spgemm_descr* d, PhasePlan* plans[], int n,
   return RVSP_SUCCESS;
 }
 
-int bin_rows(const PhasePlan& p, int b) { return p.offsets[b + 1] - p.offsets[b]; }
 
 // Finds the bins whose rows are one run of consecutive rows (row lists are
 // ascending and distinct, so first and last decide). Synchronous.
@@ -634,13 +633,14 @@ rvsp_status_t run_numeric(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B
                           const int* c_ci, float* c_val, hipStream_t s) {
   const RowStats st = row_stats(d);
   for (int b = 1; b < d->numeric.table.rule.nbins; ++b)
-    if (d->numeric.table.spec[b].kind == KIND_BLOCK && bin_rows(d->numeric, b) > 0) {
+    if (d->numeric.table.spec[b].kind == KIND_BLOCK && bin_rows(d->numeric, b) > 0 &&
+        !((d->stream_bins >> b) & 1u)) {
       RVSP_TRY(ensure_split(d, B, s));
       break;
     }
   for (int b = 1; b < d->numeric.table.rule.nbins; ++b) {
     const int n = bin_rows(d->numeric, b);
-    if (n == 0) continue;
+    if (n == 0 || ((d->stream_bins >> b) & 1u)) continue;
     const BinSpec& spec = d->numeric.table.spec[b];
     const int* rows = (const int*)d->numeric.rows.p + d->numeric.offsets[b];
     if (d->defer_block && spec.kind == KIND_BLOCK && spec.variant == 0) {
@@ -748,7 +748,7 @@ rvsp_status_t promote_to_map(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr
   PhaseTable& t = d->numeric.table;
   bool any = false;
   for (int b = 2; b < t.rule.nbins; ++b) {
-    if (bin_rows(d->numeric, b) == 0) continue;
+    if (bin_rows(d->numeric, b) == 0 || ((d->stream_bins >> b) & 1u)) continue;
     // Searching bins keep their shape; tiled bins of rows short enough for
     // a workgroup-sized accumulator take the widest map shape.
     const bool block = t.spec[b].kind == KIND_BLOCK && t.rule.bound[b] <= kMapFromTiledMax;

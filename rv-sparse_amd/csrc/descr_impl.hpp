@@ -143,6 +143,14 @@ struct rvsp_spgemm_descr {
   bool defer_block = false;
   std::vector<rvsp::DeferredBin> deferred;
 
+  // Set on a batch's inner descriptors: numeric bins served by the batch's
+  // persistent row-stream launch (run_numeric skips them, and they never move
+  // to the position map).
+  unsigned stream_bins = 0;
+  // Batch setting: numeric-only computes run every group's short-row numeric
+  // bins as one persistent row-stream launch.
+  bool batch_persistent = true;
+
   // Set by the one-shot path: work_estimation checks the staged operands
   // for canonical CSR on the device before any other kernel reads them.
   bool check_csr = false;
@@ -205,6 +213,9 @@ rvsp_status_t repartition_phase(rvsp_spgemm_descr* d, int phase, const DevCsr& A
 int rule_len_class(int b_rows, int b_nnz);
 // The tuned rule parameter of a phase for that key (apply_rule), or -1.
 int tuned_rule(int phase, int key);
+
+// Rows of bin b of a partitioned phase.
+inline int bin_rows(const PhasePlan& p, int b) { return p.offsets[b + 1] - p.offsets[b]; }
 
 // row_base of the numeric bin whose row list starts at `rows` (-1 if none).
 int numeric_row_base(const rvsp_spgemm_descr* d, const int* rows);

@@ -32,13 +32,6 @@ namespace rvsp {
 // flattened products are dealt out round-robin (load-balanced search), so a
 // group stays busy whatever the mix of long and short B rows. Order of the
 // calls is unspecified: only for order-free work (count, fill).
-// Keeps the compiler from moving LDS accesses across this point (the
-// hardware already executes one wave's LDS operations in program order), so
-// values written by other lanes of the wave are read from LDS.
-__device__ __forceinline__ void wave_lds_order() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-}
-
 template <int G, class F>
 __device__ __forceinline__ void group_for_each_col(int a_s, int a_e,
                                                    const int* __restrict__ a_ci,
@@ -1288,6 +1281,16 @@ __global__ __launch_bounds__(kBlock) void k_scan64_apply(const long long* __rest
   }
 }
 
+hipError_t launch_scan64(const long long* in, int n, long long* out, long long* tile_sums,
+                         long long* total, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int ntiles = (n + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_scan64_reduce, dim3(ntiles), dim3(kBlock), 0, s, in, n, tile_sums);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, s, tile_sums, ntiles, total);
+  hipLaunchKernelGGL(k_scan64_apply, dim3(ntiles), dim3(kBlock), 0, s, in, n, tile_sums, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_map_offsets(int M, const unsigned char* bin_of, unsigned bin_mask, RowStats st,
                               long long* weights, long long* poff, long long* tile_sums,
                               long long* total, hipStream_t s, bool span_bound) {
@@ -2183,17 +2186,6 @@ constexpr int kWaveHashSpan = 1 << (31 - kWavePosBits);  // hash tiles' column s
 int wave_tile_lds_bytes(int Tw) {
   // vals[Tw] | index (Tw uint2 words, or 2 Tw hash slots) | tags | marks
   return (3 * Tw + kWaveTags + kWave) * 4;
-}
-
-template <int G>
-__device__ __forceinline__ int group_incl_max(int v) {
-  const int l = threadIdx.x & (G - 1);
-#pragma unroll
-  for (int d = 1; d < G; d <<= 1) {
-    const int t = __shfl_up(v, d, G);
-    if (l >= d) v = max(v, t);
-  }
-  return v;
 }
 
 __global__ __launch_bounds__(kWave) void k_numeric_wave(

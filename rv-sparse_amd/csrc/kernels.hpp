@@ -251,6 +251,74 @@ hipError_t launch_numeric_map_bin(const BinSpec& b, const MapView& mv, const int
                                   int nrows, const DevCsr& A, const DevCsr& B,
                                   const int* c_rp, float* c_val, hipStream_t s);
 
+// --- persistent row-stream numeric pass (stream.hip) ------------------------
+// Numeric bins of one or several descriptors (a batch's groups) served by one
+// persistent launch. A segment is one bin: its rows in list order, whose
+// products form one stream (soff[i] = products of list positions before i).
+// The stream is cut into queue items of kStreamItem products; an item owns
+// the rows whose first product lies in it (item_first[k] = first such list
+// position, item_first[items] = nrows). Each wave takes items from a queue
+// and walks its rows' products 64 at a time across A entries. A map segment
+// reads every product's output position from `map` (one 16-bit entry per
+// product of the stream; bits 0-10 the position in the C row, bit 11 set on
+// the row's first product, bits 12-15 the product's order among earlier
+// products of the same C entry in the same 64-product window, 15 = apply in
+// lane order); an identity segment (rows fed by one B row, map null) needs
+// no map. Rows of map segments hold at most kStreamMaxRow entries.
+struct StreamSeg {
+  const int* rows = nullptr;  // list position -> row, or null: row_base + i
+  const int* a_rp = nullptr;
+  const int* c_rp = nullptr;
+  const int* c_ci = nullptr;  // read when building the map
+  float* c_val = nullptr;
+  const long long* soff = nullptr;  // nrows + 1
+  const int* item_first = nullptr;  // items + 1
+  unsigned short* map = nullptr;    // null: identity segment
+  long long total = 0;              // products of the stream (map entries)
+  int item0 = 0;                    // first queue item of the segment
+  int nrows = 0;
+  int row_base = 0;
+  int c_nnz = 0;                    // entries of the C arrays the rows index
+};
+// Work queue of one launch; next and done are zero before the first launch,
+// and every launch leaves them zero again (its last wave resets them).
+// `bad` collects kStreamBad* bits of writes the kernels refused because
+// their index fell outside the arrays (never set when segments and map
+// match the descriptors' structure).
+struct StreamQueue {
+  unsigned next;
+  unsigned done;
+  unsigned bad;
+  unsigned pad;
+};
+constexpr unsigned kStreamBadValue = 1u;   // C value outside the segment's C arrays
+constexpr unsigned kStreamBadMap = 2u;     // map entry outside the stream
+constexpr unsigned kStreamBadLds = 4u;     // accumulator outside the sub-batch
+constexpr unsigned kStreamBadColumn = 8u;  // product column not in its C row
+constexpr int kStreamItem = 8192;
+constexpr int kStreamMaxRow = 2048;
+constexpr int kStreamMaxSegs = 256;
+constexpr int kStreamPosBits = 11;
+// w[i] = ops[row of list position i].
+hipError_t launch_stream_weights(const int* rows, int row_base, int nrows, const long long* ops,
+                                 long long* w, hipStream_t s);
+// Exclusive scan of n int64 values into out[0..n] (out[n] = *total);
+// tile_sums holds scan_tile_count(n) values.
+hipError_t launch_scan64(const long long* in, int n, long long* out, long long* tile_sums,
+                         long long* total, hipStream_t s);
+hipError_t launch_stream_items(const long long* soff, int nrows, int items, int* item_first,
+                               hipStream_t s);
+// Waves of a persistent launch: as many as the device holds at once (one
+// wave per workgroup), at most `items`.
+int stream_grid(bool build, int capw, int items);
+// build: every map segment's entries from C's columns (positions and row
+// marks); otherwise the numeric pass. capw: the longest row of any map
+// segment; grid from stream_grid.
+hipError_t launch_stream(bool build, const StreamSeg* segs, int nseg, int items, const DevCsr& A,
+                         const DevCsr& B, StreamQueue* q, int capw, int grid, hipStream_t s);
+// Order bits of a map segment's entries (after the build).
+hipError_t launch_stream_rounds(unsigned short* map, long long total, hipStream_t s);
+
 // One-time kernel attribute setup: allows `lds_bytes` of dynamic LDS.
 hipError_t init_kernel_attributes(int lds_bytes);
 

@@ -1,0 +1,409 @@
+// stream.hip — persistent row-stream numeric pass.
+//
+// One launch serves the numeric bins of one or several descriptors (all
+// groups of a batch). A bin is a segment: its rows in list order, whose
+// products, laid end to end, form the segment's stream. The streams are cut
+// into queue items of kStreamItem products; every wave of the launch takes
+// items from one queue until none is left, so the work is balanced by
+// products across all bins and groups, and the launch has no per-bin tail.
+//
+// A wave walks the rows of its item in sub-batches (up to 64 rows whose C
+// rows fit the wave's LDS accumulators) and their products 64 at a time,
+// across A entries and rows: lane l of a step takes product 64 w + l of the
+// stream (window w), finds its A entry by the entry marks of the step and a
+// running maximum, and gathers one B value. Its output position comes from
+// the segment's map (built once per structure by the same traversal, with a
+// binary search of the product's column in the C row's sorted columns).
+//
+// Order. The reference adds the products of C(i,j) in A order, one rounded
+// add at a time (gustavson_core_f32.inc:18-64, accum_scalar_f32.c:67-70). A
+// step holds products of several A entries, so two of its lanes may hit the
+// same C entry. Every map entry carries its order among the earlier products
+// of the same C entry in the same window (computed once, k_stream_rounds);
+// a step applies its lanes in rounds of that order, which is one round when
+// no C entry repeats in the window. Lanes of one round have distinct
+// positions; windows follow each other in stream order, and a row belongs
+// to one wave. The result is bit-identical to every other numeric kernel.
+#include <climits>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace rvsp {
+
+namespace {
+
+constexpr unsigned kPosMask = (1u << kStreamPosBits) - 1u;
+constexpr unsigned kRowMark = 1u << kStreamPosBits;
+constexpr int kOrderShift = 12;
+constexpr unsigned kLaneOrder = 15;  // order value: apply in lane order
+
+__global__ __launch_bounds__(kBlock) void k_stream_weights(const int* __restrict__ rows,
+                                                           int row_base, int nrows,
+                                                           const long long* __restrict__ ops,
+                                                           long long* __restrict__ w) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nrows) return;
+  w[i] = ops[rows ? rows[i] : row_base + i];
+}
+
+// first[k] = the first list position whose stream offset is >= k * item,
+// for k in [0, items]; first[items] = nrows.
+__global__ __launch_bounds__(kBlock) void k_stream_items(const long long* __restrict__ soff,
+                                                         int nrows, int items,
+                                                         int* __restrict__ first) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i > nrows) return;
+  const long long lo = i > 0 ? soff[i - 1] : -1;
+  const long long k0 = lo < 0 ? 0 : lo / kStreamItem + 1;
+  long long k1 = i == nrows ? items : soff[i] / kStreamItem;
+  if (k1 > items) k1 = items;
+  for (long long k = k0; k <= k1; ++k) first[k] = i;
+}
+
+// Lanes with `has` whose `start` lies in [base, base + 64) mark that slot
+// with their lane id; every lane returns the last mark at or before its own
+// slot, or `carry` (the owner of the previous window's last slot).
+__device__ __forceinline__ int wave_owner(int* mark, bool has, int start, int base, int carry) {
+  const int lane = threadIdx.x & (kWave - 1);
+  mark[lane] = -1;
+  wave_lds_order();
+  if (has && start >= base && start < base + kWave) mark[start - base] = lane;
+  wave_lds_order();
+  const int m = mark[lane];
+  wave_lds_order();
+  return max(group_incl_max<kWave>(m), carry);
+}
+
+// One step's adds into the accumulators, in the rounds of the map's order
+// bits (lanes of one round have distinct positions).
+__device__ __forceinline__ void stream_apply(float* vals, bool pv, int at, unsigned ord, float a,
+                                             float b) {
+  if (__ballot(pv && ord != 0) == 0) {
+    if (pv) vals[at] = acc_step(vals[at], a, b);
+    return;
+  }
+  for (unsigned r = 0; r < kLaneOrder; ++r) {
+    if (__ballot(pv && ord >= r) == 0) return;
+    if (pv && ord == r) vals[at] = acc_step(vals[at], a, b);
+    wave_lds_order();
+  }
+  const int lane = threadIdx.x & (kWave - 1);
+  unsigned long long left = __ballot(pv && ord == kLaneOrder);
+  while (left) {
+    const int l = __ffsll((long long)left) - 1;
+    if (lane == l) vals[at] = acc_step(vals[at], a, b);
+    wave_lds_order();
+    left &= left - 1;
+  }
+}
+
+template <bool BUILD>
+__global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ segs, int nseg,
+                                                  int items, DevCsr A, DevCsr B,
+                                                  StreamQueue* __restrict__ q, int capw) {
+  extern __shared__ __attribute__((aligned(16))) int st_lds[];
+  float* vals = reinterpret_cast<float*>(st_lds);  // numeric: accumulators of the sub-batch
+  int* cols = st_lds;                               // build: C columns of the sub-batch
+  int* mark = st_lds + capw;
+  const int lane = threadIdx.x;
+  constexpr int kSegLanes = kStreamMaxSegs / kWave;
+  unsigned bad = 0;  // kStreamBad* bits of refused writes
+  int seg_i0[kSegLanes];
+#pragma unroll
+  for (int j = 0; j < kSegLanes; ++j) {
+    const int sg = lane + j * kWave;
+    seg_i0[j] = sg < nseg ? segs[sg].item0 : INT_MAX;
+  }
+  for (;;) {
+    int item = 0;
+    if (lane == 0) item = (int)atomicAdd(&q->next, 1u);
+    item = __builtin_amdgcn_readfirstlane(item);
+    if (item >= items) break;
+    int sidx = -1;
+#pragma unroll
+    for (int j = 0; j < kSegLanes; ++j) sidx += __popcll(__ballot(seg_i0[j] <= item));
+    sidx = __builtin_amdgcn_readfirstlane(sidx);
+    const StreamSeg& S = segs[sidx];
+    const bool ident = S.map == nullptr;
+    if (BUILD && ident) continue;
+    const int* __restrict__ rows = S.rows;
+    const int row_base = S.row_base;
+    const int* __restrict__ a_rp = S.a_rp;
+    const int* __restrict__ c_rp = S.c_rp;
+    const long long* __restrict__ soff = S.soff;
+    unsigned short* __restrict__ map = S.map;
+    const long long total = S.total;
+    const int c_nnz = S.c_nnz;
+    const int k = item - S.item0;
+    int i = S.item_first[k];
+    const int i_end = S.item_first[k + 1];
+    while (i < i_end) {
+      // One lane per row of the sub-batch.
+      const int ii = i + lane;
+      const bool rv = ii < i_end;
+      int row = 0, cs = 0, n = 0, as = 0, ae = 0;
+      long long rq = 0;
+      if (rv) {
+        row = rows ? rows[ii] : row_base + ii;
+        cs = c_rp[row];
+        n = c_rp[row + 1] - cs;
+        as = a_rp[row];
+        ae = a_rp[row + 1];
+        rq = soff[ii];
+      }
+      const int incl_n = group_incl_scan<kWave>(n);
+      int R = ident ? min(kWave, i_end - i) : __popcll(__ballot(rv && incl_n <= capw));
+      R = __builtin_amdgcn_readfirstlane(R);
+      if (R == 0) {
+        // A row longer than the accumulators (the host never plans one):
+        // skipped rather than overrun.
+        ++i;
+        continue;
+      }
+      const bool rl = lane < R;
+      if (!rl) {
+        n = 0;
+        as = ae = 0;
+      }
+      const int abase = incl_n - n;  // accumulator offset of the row (rows < R)
+      const int totn = ident ? 0 : __shfl(incl_n, R - 1);
+      const int ne = ae - as;
+      const int incl_e = group_incl_scan<kWave>(ne);
+      const int ebase = incl_e - ne;
+      const int E = __shfl(incl_e, kWave - 1);
+      if (!ident) {
+        // Accumulators cleared (numeric) or the rows' columns staged (build).
+        int rc = 0;
+        for (int j0 = 0; j0 < totn; j0 += kWave) {
+          if (!BUILD) {
+            if (j0 + lane < totn) vals[j0 + lane] = 0.0f;
+            continue;
+          }
+          const int r = wave_owner(mark, rl && n > 0, abase, j0, rc);
+          rc = __shfl(r, kWave - 1);
+          const int src = __shfl(cs, r) + j0 + lane - __shfl(abase, r);
+          if (j0 + lane < totn) cols[j0 + lane] = S.c_ci[src];
+        }
+        wave_lds_order();
+      }
+      long long qc = __shfl(rq, 0);  // stream index of the chunk's first product
+      int ec = 0;
+      for (int c0 = 0; c0 < E; c0 += kWave) {
+        // One lane per A entry of the chunk.
+        const int er = wave_owner(mark, rl && ne > 0, ebase, c0, ec);
+        ec = __shfl(er, kWave - 1);
+        const int eidx = c0 + lane;
+        const bool ev = eidx < E;
+        const int ap = __shfl(as, er) + eidx - __shfl(ebase, er);
+        const int e_acc = __shfl(abase, er);
+        const int e_n = __shfl(n, er);
+        const int e_cs = __shfl(cs, er);
+        const long long e_rq = __shfl(rq, er);
+        int bs = 0, len = 0;
+        float a = 0.0f;
+        if (ev) {
+          const int kk = A.ci[ap];
+          a = A.val[ap];
+          bs = B.rp[kk];
+          len = B.rp[kk + 1] - bs;
+        }
+        const int incl_p = group_incl_scan<kWave>(len);
+        const int excl_p = incl_p - len;
+        const int boff = bs - excl_p;  // B position of the entry's product at chunk offset t: boff + t
+        const int T = __shfl(incl_p, kWave - 1);
+        if (T > 0) {
+          const long long w0 = qc >> 6, w1 = (qc + T - 1) >> 6;
+          int pc = 0;
+          // Entry, B position and operands of every lane of window w.
+          auto locate = [&](long long w, bool& pv, int& bi, float& av, int& p_acc, int& p_n,
+                            int& p_cs, long long& p_rq) {
+            const int off = (int)((w << 6) - qc);
+            const int e = wave_owner(mark, len > 0, excl_p, off, pc);
+            pc = __shfl(e, kWave - 1);
+            const int t = off + lane;
+            pv = t >= 0 && t < T;
+            bi = __shfl(boff, e) + t;
+            av = __shfl(a, e);
+            p_acc = __shfl(e_acc, e);
+            p_n = __shfl(e_n, e);
+            p_cs = __shfl(e_cs, e);
+            p_rq = __shfl(e_rq, e);
+          };
+          if (BUILD) {
+            for (long long w = w0; w <= w1; ++w) {
+              bool pv;
+              int bi, p_acc, p_n, p_cs;
+              float av;
+              long long p_rq;
+              locate(w, pv, bi, av, p_acc, p_n, p_cs, p_rq);
+              const long long qq = (w << 6) + lane;
+              if (pv) {
+                const int col = B.ci[bi];
+                const unsigned pos = (unsigned)lds_find(cols + p_acc, p_n, col);
+                if (cols[p_acc + pos] != col || pos > kPosMask) bad |= kStreamBadColumn;
+                if (qq < 0 || qq >= total) bad |= kStreamBadMap;
+                else map[qq] = (unsigned short)(pos | (qq == p_rq ? kRowMark : 0u));
+              }
+            }
+          } else if (ident) {
+            for (long long w = w0; w <= w1; ++w) {
+              bool pv;
+              int bi, p_acc, p_n, p_cs;
+              float av;
+              long long p_rq;
+              locate(w, pv, bi, av, p_acc, p_n, p_cs, p_rq);
+              const long long qq = (w << 6) + lane;
+              const long long dst = (long long)p_cs + (qq - p_rq);
+              if (pv && (dst < 0 || dst >= c_nnz)) bad |= kStreamBadValue;
+              else if (pv) S.c_val[dst] = acc_step(0.0f, av, B.val[bi]);
+            }
+          } else {
+            // The next window's map entry and B value are in flight while
+            // this window is applied.
+            bool pv0, pv1;
+            int bi, acc0, acc1, p_n, p_cs;
+            float av0, av1, bv0, bv1;
+            unsigned me0, me1;
+            long long p_rq;
+            locate(w0, pv0, bi, av0, acc0, p_n, p_cs, p_rq);
+            me0 = pv0 ? map[(w0 << 6) + lane] : 0u;
+            bv0 = pv0 ? B.val[bi] : 0.0f;
+            for (long long w = w0; w <= w1; ++w) {
+              if (w < w1) {
+                locate(w + 1, pv1, bi, av1, acc1, p_n, p_cs, p_rq);
+                me1 = pv1 ? map[((w + 1) << 6) + lane] : 0u;
+                bv1 = pv1 ? B.val[bi] : 0.0f;
+              }
+              const int at = acc0 + (int)(me0 & kPosMask);
+              const bool ok = !pv0 || (at >= 0 && at < totn);
+              if (!ok) bad |= kStreamBadLds;
+              stream_apply(vals, pv0 && ok, at, me0 >> kOrderShift, av0, bv0);
+              pv0 = pv1;
+              acc0 = acc1;
+              av0 = av1;
+              bv0 = bv1;
+              me0 = me1;
+            }
+          }
+        }
+        qc += T;
+      }
+      if (!BUILD && !ident) {
+        // Accumulators to C, row by row in order (coalesced runs).
+        wave_lds_order();
+        int rc = 0;
+        for (int j0 = 0; j0 < totn; j0 += kWave) {
+          const int r = wave_owner(mark, rl && n > 0, abase, j0, rc);
+          rc = __shfl(r, kWave - 1);
+          const int dst = __shfl(cs, r) + j0 + lane - __shfl(abase, r);
+          if (j0 + lane < totn) {
+            if (dst < 0 || dst >= c_nnz) bad |= kStreamBadValue;
+            else S.c_val[dst] = vals[j0 + lane];
+          }
+        }
+        wave_lds_order();
+      }
+      i += R;
+    }
+  }
+  unsigned all = bad;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) all |= __shfl_xor(all, d);
+  if (lane == 0 && all) atomicOr(&q->bad, all);
+  // The last wave out leaves the queue at zero for the next launch.
+  if (lane == 0) {
+    const unsigned d = atomicAdd(&q->done, 1u);
+    if (d == gridDim.x - 1) {
+      atomicExch(&q->next, 0u);
+      atomicExch(&q->done, 0u);
+    }
+  }
+}
+
+// Order bits of one segment's map: one wave per window of 64 stream
+// positions; rows within the window are told apart by the row marks.
+__global__ __launch_bounds__(kBlock) void k_stream_rounds(unsigned short* __restrict__ map,
+                                                          long long total) {
+  const long long qq = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const bool v = qq < total;
+  const unsigned ent = v ? map[qq] : 0u;
+  const int rowid = group_incl_scan<kWave>((int)((ent >> kStreamPosBits) & 1u));
+  const int key = v ? (int)(((unsigned)rowid << kStreamPosBits) | (ent & kPosMask)) : -1 - lane;
+  int before = 0;
+#pragma unroll 8
+  for (int d = 1; d < kWave; ++d) {
+    const int o = __shfl_up(key, d);
+    if (lane >= d && o == key) ++before;
+  }
+  const unsigned ord = before < (int)kLaneOrder ? (unsigned)before : kLaneOrder;
+  if (v) map[qq] = (unsigned short)((ent & (kPosMask | kRowMark)) | (ord << kOrderShift));
+}
+
+int stream_waves_per_cu(bool build, int lds) {
+  int nb = 0;
+  const hipError_t e = build ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_stream<true>,
+                                                                            kWave, lds)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_stream<false>,
+                                                                            kWave, lds);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return 8;
+  }
+  return nb > 0 ? nb : 1;
+}
+
+}  // namespace
+
+hipError_t launch_stream_weights(const int* rows, int row_base, int nrows, const long long* ops,
+                                 long long* w, hipStream_t s) {
+  if (nrows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_stream_weights, dim3((nrows + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                     rows, row_base, nrows, ops, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_stream_items(const long long* soff, int nrows, int items, int* item_first,
+                               hipStream_t s) {
+  if (nrows < 0 || items <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_stream_items, dim3(nrows / kBlock + 1), dim3(kBlock), 0, s, soff, nrows,
+                     items, item_first);
+  return hipGetLastError();
+}
+
+int stream_grid(bool build, int capw, int items) {
+  if (items <= 0 || capw < 0 || capw > kStreamMaxRow) return 0;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    cus = 256;
+  }
+  const long long fit = (long long)stream_waves_per_cu(build, (capw + kWave) * 4) * cus;
+  return (int)(fit < items ? fit : items);
+}
+
+hipError_t launch_stream(bool build, const StreamSeg* segs, int nseg, int items, const DevCsr& A,
+                         const DevCsr& B, StreamQueue* q, int capw, int grid, hipStream_t s) {
+  if (items <= 0 || nseg <= 0) return hipSuccess;
+  if (nseg > kStreamMaxSegs || capw < 0 || capw > kStreamMaxRow || grid <= 0)
+    return hipErrorInvalidValue;
+  const int lds = (capw + kWave) * 4;
+  if (build)
+    hipLaunchKernelGGL(k_stream<true>, dim3(grid), dim3(kWave), lds, s, segs, nseg, items, A, B,
+                       q, capw);
+  else
+    hipLaunchKernelGGL(k_stream<false>, dim3(grid), dim3(kWave), lds, s, segs, nseg, items, A,
+                       B, q, capw);
+  return hipGetLastError();
+}
+
+hipError_t launch_stream_rounds(unsigned short* map, long long total, hipStream_t s) {
+  if (total <= 0) return hipSuccess;
+  const long long blocks = (total + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_stream_rounds, dim3((unsigned)blocks), dim3(kBlock), 0, s, map, total);
+  return hipGetLastError();
+}
+
+}  // namespace rvsp

@@ -136,6 +136,8 @@ PROTOTYPES = {
     "rvsp_spgemm_batch_set_graph": (C.c_int, [P, C.c_int]),
     "rvsp_spgemm_batch_set_group_limit": (C.c_int, [P, I64]),
     "rvsp_spgemm_batch_set_fuse": (C.c_int, [P, C.c_int]),
+    "rvsp_spgemm_batch_set_persistent": (C.c_int, [P, C.c_int]),
+    "rvsp_spgemm_batch_stream_segments": (I32, [P]),
     "rvsp_hip_malloc": (C.c_int, [C.POINTER(P), SZ]),
     "rvsp_hip_free": (C.c_int, [P]),
     "rvsp_hip_memcpy": (C.c_int, [P, P, SZ, C.c_int]),

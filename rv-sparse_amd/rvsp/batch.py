@@ -13,7 +13,7 @@ class Batch:
     """A descriptor used for batched products C[p] = A[p] * B[p]."""
 
     def __init__(self, algo=F.ALGO_DEFAULT, group_limit=None, graph=True, choices=None,
-                 fuse=False):
+                 fuse=False, persistent=True):
         self.handle = C.c_void_p()
         check(lib().rvsp_spgemm_descr_create(C.byref(self.handle)), "descr_create")
         check(lib().rvsp_spgemm_set_algo(self.handle, algo), "set_algo")
@@ -27,6 +27,8 @@ class Batch:
                   "batch_set_group_limit")
         check(lib().rvsp_spgemm_batch_set_graph(self.handle, 1 if graph else 0), "set_graph")
         check(lib().rvsp_spgemm_batch_set_fuse(self.handle, 1 if fuse else 0), "set_fuse")
+        check(lib().rvsp_spgemm_batch_set_persistent(self.handle, 1 if persistent else 0),
+              "set_persistent")
         self.c_nnz = None
 
     @staticmethod
@@ -54,6 +56,9 @@ class Batch:
 
     def graph_launches(self):
         return int(lib().rvsp_spgemm_batch_graph_launches(self.handle))
+
+    def stream_segments(self):
+        return int(lib().rvsp_spgemm_batch_stream_segments(self.handle))
 
     def stream(self):
         s = C.c_void_p()
@@ -130,10 +135,11 @@ def upload_pairs(pairs):
 
 
 def spgemm_batch(pairs, algo=F.ALGO_DEFAULT, contiguous=True, group_limit=None, graph=True,
-                 repeats=1, choices=None, fuse=False):
+                 repeats=1, choices=None, fuse=False, persistent=True):
     """Host pairs in, host products out, through the batched device path."""
     As, Bs, owned = upload_pairs(pairs)
-    bt = Batch(algo=algo, group_limit=group_limit, graph=graph, choices=choices, fuse=fuse)
+    bt = Batch(algo=algo, group_limit=group_limit, graph=graph, choices=choices, fuse=fuse,
+               persistent=persistent)
     out = None
     try:
         nnz = bt.estimate(As, Bs)
