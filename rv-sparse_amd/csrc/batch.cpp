@@ -117,7 +117,7 @@ struct BatchState {
   // bins (numeric-only computes): per segment its list-order product offsets,
   // queue-item starts and map; the segment table and the launch's queue.
   struct StreamPart {
-    DevBuf soff, items, map;
+    DevBuf rows, soff, items, map;
   };
   std::vector<StreamPart> parts;
   std::vector<StreamSeg> segs;
@@ -182,6 +182,7 @@ struct BatchState {
   }
   void drop_stream() {
     for (StreamPart& sp : parts) {
+      release(sp.rows);
       release(sp.soff);
       release(sp.items);
       release(sp.map);
@@ -364,42 +365,89 @@ void build_output_tables(BatchState& bs, const rvsp_csr_matrix_t* C) {
 rvsp_status_t build_stream(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream_t s) {
   bs.drop_stream();
   bs.stream_failed = true;  // until every step below has succeeded
+  // Per group up to two segments: the rows fed by one B row (bin 1, no map),
+  // and every other row of a bin whose rows are at most kStreamMaxRow long,
+  // in ascending row order (one pair's rows next to each other, so the
+  // waves walking them read the same B).
   struct Pick {
-    int group, bin, nrows, row_base;
+    int group, nrows, row_base;
     const int* rows;
     bool ident;
+    unsigned bins;
   };
   std::vector<Pick> picks;
-  int max_rows = 0;
-  for (size_t gi = 0; gi < bs.groups.size(); ++gi) {
+  int max_rows = 0, max_group_rows = 0;
+  for (size_t gi = 0; gi < bs.groups.size() && (int)picks.size() + 2 <= kStreamMaxSegs; ++gi) {
     const rvsp_spgemm_descr* in = bs.groups[gi].inner;
     const PhasePlan& p = in->numeric;
-    for (int b = 1; b < p.table.rule.nbins; ++b) {
-      const int n = bin_rows(p, b);
-      if (n == 0) continue;
-      const bool ident = b == 1;
-      // Rows of bin b are at most bound[b] long, except in the last bin,
-      // which takes every row no other bin admits.
-      if (!ident && (b == p.table.rule.nbins - 1 || p.table.rule.bound[b] > kStreamMaxRow))
-        continue;
-      if ((int)picks.size() == kStreamMaxSegs) break;
+    const int M = (int)(bs.groups[gi].r1 - bs.groups[gi].r0);
+    if (bin_rows(p, 1) > 0) {
       Pick pk;
       pk.group = (int)gi;
-      pk.bin = b;
-      pk.nrows = n;
-      pk.rows = (const int*)p.rows.p + p.offsets[b];
-      pk.row_base = (size_t)b < p.row_base.size() ? p.row_base[b] : -1;
-      pk.ident = ident;
+      pk.nrows = bin_rows(p, 1);
+      pk.rows = (const int*)p.rows.p + p.offsets[1];
+      pk.row_base = p.row_base.size() > 1 ? p.row_base[1] : -1;
+      pk.ident = true;
+      pk.bins = 1u << 1;
       picks.push_back(pk);
-      max_rows = std::max(max_rows, n);
+      max_rows = std::max(max_rows, pk.nrows);
+    }
+    Pick pk;
+    pk.group = (int)gi;
+    pk.nrows = 0;
+    pk.rows = nullptr;
+    pk.row_base = -1;
+    pk.ident = false;
+    pk.bins = 0;
+    // Rows of bin b are at most bound[b] long, except in the last bin,
+    // which takes every row no other bin admits.
+    for (int b = 2; b < p.table.rule.nbins - 1; ++b)
+      if (bin_rows(p, b) > 0 && p.table.rule.bound[b] <= kStreamMaxRow) {
+        pk.bins |= 1u << b;
+        pk.nrows += bin_rows(p, b);
+      }
+    if (pk.bins) {
+      picks.push_back(pk);
+      max_rows = std::max(max_rows, pk.nrows);
+      max_group_rows = std::max(max_group_rows, M);
     }
   }
   if (picks.empty()) return RVSP_SUCCESS;
+  // Row lists of the merged segments: rows of their bins, in row order.
+  bs.parts.resize(picks.size());
+  if (max_group_rows > 0) {
+    DevBuf flags, pos;
+    RVSP_TRY(ensure(flags, (size_t)max_group_rows * 4));
+    RVSP_TRY(ensure(pos, ((size_t)max_group_rows + 1) * 4));
+    RVSP_TRY(ensure(bs.s_tiles, scan_tile_count(std::max(max_rows, max_group_rows)) * 8));
+    RVSP_TRY(ensure(bs.s_totals, picks.size() * 8));
+    for (size_t j = 0; j < picks.size(); ++j) {
+      Pick& pk = picks[j];
+      if (pk.ident) continue;
+      const rvsp_spgemm_descr* in = bs.groups[pk.group].inner;
+      const PhasePlan& p = in->numeric;
+      const int M = (int)(bs.groups[pk.group].r1 - bs.groups[pk.group].r0);
+      RVSP_HIP_TRY(hipMemsetAsync(flags.p, 0, (size_t)M * 4, s));
+      for (int b = 2; b < p.table.rule.nbins; ++b)
+        if ((pk.bins >> b) & 1u)
+          RVSP_HIP_TRY(launch_stream_mark((const int*)p.rows.p + p.offsets[b],
+                                          (size_t)b < p.row_base.size() ? p.row_base[b] : -1,
+                                          bin_rows(p, b), (int*)flags.p, s));
+      RVSP_HIP_TRY(launch_scan((const int*)flags.p, M, (int*)pos.p,
+                               (long long*)bs.s_totals.p + j, (long long*)bs.s_tiles.p, s));
+      RVSP_TRY(ensure(bs.parts[j].rows, (size_t)pk.nrows * 4));
+      RVSP_HIP_TRY(launch_stream_compact((const int*)flags.p, (const int*)pos.p, M,
+                                         (int*)bs.parts[j].rows.p, s));
+      pk.rows = (const int*)bs.parts[j].rows.p;
+    }
+    RVSP_HIP_TRY(hipStreamSynchronize(s));
+    release_idle(flags);
+    release_idle(pos);
+  }
   const int ns = (int)picks.size();
   RVSP_TRY(ensure(bs.s_weights, (size_t)max_rows * 8));
   RVSP_TRY(ensure(bs.s_tiles, scan_tile_count(max_rows) * 8));
   RVSP_TRY(ensure(bs.s_totals, (size_t)ns * 8));
-  bs.parts.resize(ns);
   for (int j = 0; j < ns; ++j) {
     const Pick& pk = picks[j];
     RVSP_TRY(ensure(bs.parts[j].soff, ((size_t)pk.nrows + 1) * 8));
@@ -420,7 +468,9 @@ rvsp_status_t build_stream(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream
   for (int j = 0; j < ns; ++j) {
     if (!picks[j].ident) {
       map_bytes += (size_t)totals[j] * 2;
-      capw = std::max(capw, bs.groups[picks[j].group].inner->numeric.table.rule.bound[picks[j].bin]);
+      const PhasePlan& p = bs.groups[picks[j].group].inner->numeric;
+      for (int b = 2; b < p.table.rule.nbins; ++b)
+        if ((picks[j].bins >> b) & 1u) capw = std::max(capw, p.table.rule.bound[b]);
     }
     items += (totals[j] + kStreamItem - 1) / kStreamItem;
   }
@@ -449,7 +499,8 @@ rvsp_status_t build_stream(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream
     RVSP_TRY(ensure(part.items, ((size_t)nitems + 1) * 4));
     RVSP_HIP_TRY(launch_stream_items((const long long*)part.soff.p, pk.nrows, nitems,
                                      (int*)part.items.p, s));
-    if (!pk.ident && totals[j] > 0) RVSP_TRY(ensure(part.map, (size_t)totals[j] * 2));
+    // 128 entries of slack: the numeric pass loads whole windows, one ahead.
+    if (!pk.ident && totals[j] > 0) RVSP_TRY(ensure(part.map, (size_t)totals[j] * 2 + 256));
     StreamSeg& sg = bs.segs[j];
     sg.rows = pk.row_base >= 0 ? nullptr : pk.rows;
     sg.row_base = pk.row_base >= 0 ? pk.row_base : 0;
@@ -485,7 +536,7 @@ rvsp_status_t build_stream(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream
   bs.stream_items = item0;
   bs.stream_capw = capw;
   bs.stream_grid = stream_grid(false, capw, item0);
-  for (const Pick& pk : picks) bs.groups[pk.group].inner->stream_bins |= 1u << pk.bin;
+  for (const Pick& pk : picks) bs.groups[pk.group].inner->stream_bins |= pk.bins;
   bs.stream_failed = false;
   bs.stream_ready = true;
   return RVSP_SUCCESS;

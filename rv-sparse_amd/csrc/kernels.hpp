@@ -285,11 +285,15 @@ struct StreamSeg {
 // `bad` collects kStreamBad* bits of writes the kernels refused because
 // their index fell outside the arrays (never set when segments and map
 // match the descriptors' structure).
+// The items are split into kStreamQueues contiguous ranges, one per XCD
+// (workgroup w runs on XCD w mod 8): a wave takes items of its own XCD's
+// range first, so neighbouring rows (one pair's, reading one B) share that
+// XCD's L2, and then helps the other ranges.
+constexpr int kStreamQueues = 8;
 struct StreamQueue {
-  unsigned next;
+  unsigned next[kStreamQueues];
   unsigned done;
   unsigned bad;
-  unsigned pad;
 };
 constexpr unsigned kStreamBadValue = 1u;   // C value outside the segment's C arrays
 constexpr unsigned kStreamBadMap = 2u;     // map entry outside the stream
@@ -308,6 +312,11 @@ hipError_t launch_scan64(const long long* in, int n, long long* out, long long* 
                          long long* total, hipStream_t s);
 hipError_t launch_stream_items(const long long* soff, int nrows, int items, int* item_first,
                                hipStream_t s);
+// flags[row] = 1 for the listed rows (rows null: row_base + i).
+hipError_t launch_stream_mark(const int* rows, int row_base, int nrows, int* flags, hipStream_t s);
+// list[pos[i]] = i for rows i < M with flags[i] (pos: exclusive scan of flags).
+hipError_t launch_stream_compact(const int* flags, const int* pos, int M, int* list,
+                                 hipStream_t s);
 // Waves of a persistent launch: as many as the device holds at once (one
 // wave per workgroup), at most `items`.
 int stream_grid(bool build, int capw, int items);

@@ -61,6 +61,61 @@ __global__ __launch_bounds__(kBlock) void k_stream_items(const long long* __rest
   for (long long k = k0; k <= k1; ++k) first[k] = i;
 }
 
+__global__ __launch_bounds__(kBlock) void k_stream_mark(const int* __restrict__ rows,
+                                                        int row_base, int nrows,
+                                                        int* __restrict__ flags) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < nrows) flags[rows ? rows[i] : row_base + i] = 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_stream_compact(const int* __restrict__ flags,
+                                                           const int* __restrict__ pos, int M,
+                                                           int* __restrict__ list) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < M && flags[i]) list[pos[i]] = i;
+}
+
+// Wave-wide inclusive scans on DPP lane moves (no LDS round trips): row
+// shifts by 1, 2, 4 and 8 scan each row of 16 lanes, then the last lane of
+// rows 0 and 2 is broadcast into rows 1 and 3, and lane 31 into rows 2 and 3.
+// Lanes whose source lies outside the row keep the operation's identity.
+__device__ __forceinline__ int wave_incl_max(int v) {
+  v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x111, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x112, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x114, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x118, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x142, 0xa, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x143, 0xc, 0xf, false));
+  return v;
+}
+
+__device__ __forceinline__ int wave_incl_sum(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
+  return v;
+}
+
+__device__ __forceinline__ int wave_last(int v) { return __builtin_amdgcn_readlane(v, kWave - 1); }
+
+// Global-memory views of the segment arrays (read from the segment table,
+// so the compiler cannot tell their address space): global loads and stores
+// count only against the vector-memory counter, so waits for LDS results do
+// not also wait for them.
+template <typename T>
+using gp = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gp<const T> gload(const T* p) {
+  return (gp<const T>)p;
+}
+template <typename T>
+__device__ __forceinline__ gp<T> gstore(T* p) {
+  return (gp<T>)p;
+}
+
 // Lanes with `has` whose `start` lies in [base, base + 64) mark that slot
 // with their lane id; every lane returns the last mark at or before its own
 // slot, or `carry` (the owner of the previous window's last slot).
@@ -72,7 +127,7 @@ __device__ __forceinline__ int wave_owner(int* mark, bool has, int start, int ba
   wave_lds_order();
   const int m = mark[lane];
   wave_lds_order();
-  return max(group_incl_max<kWave>(m), carry);
+  return max(wave_incl_max(m), carry);
 }
 
 // One step's adds into the accumulators, in the rounds of the map's order
@@ -105,7 +160,10 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
   extern __shared__ __attribute__((aligned(16))) int st_lds[];
   float* vals = reinterpret_cast<float*>(st_lds);  // numeric: accumulators of the sub-batch
   int* cols = st_lds;                               // build: C columns of the sub-batch
-  int* mark = st_lds + capw;
+  int* mark = st_lds + ((capw + 3) & ~3);
+  // Per A entry of the current chunk: B position of its product at chunk
+  // offset 0, scale, and accumulator (or C) offset of its row.
+  int4* etab = reinterpret_cast<int4*>(mark + kWave);
   const int lane = threadIdx.x;
   constexpr int kSegLanes = kStreamMaxSegs / kWave;
   unsigned bad = 0;  // kStreamBad* bits of refused writes
@@ -115,11 +173,24 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
     const int sg = lane + j * kWave;
     seg_i0[j] = sg < nseg ? segs[sg].item0 : INT_MAX;
   }
+  const int x0 = blockIdx.x & (kStreamQueues - 1);
+  int tried = 0;  // queues found empty (own XCD's first)
   for (;;) {
-    int item = 0;
-    if (lane == 0) item = (int)atomicAdd(&q->next, 1u);
-    item = __builtin_amdgcn_readfirstlane(item);
-    if (item >= items) break;
+    int item = -1;
+    while (tried < kStreamQueues) {
+      const int x = (x0 + tried) & (kStreamQueues - 1);
+      const int lo = (int)((long long)items * x / kStreamQueues);
+      const int hi = (int)((long long)items * (x + 1) / kStreamQueues);
+      int t = 0;
+      if (lane == 0) t = (int)atomicAdd(&q->next[x], 1u);
+      t = __builtin_amdgcn_readfirstlane(t);
+      if (t < hi - lo) {
+        item = lo + t;
+        break;
+      }
+      ++tried;
+    }
+    if (item < 0) break;
     int sidx = -1;
 #pragma unroll
     for (int j = 0; j < kSegLanes; ++j) sidx += __popcll(__ballot(seg_i0[j] <= item));
@@ -127,17 +198,19 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
     const StreamSeg& S = segs[sidx];
     const bool ident = S.map == nullptr;
     if (BUILD && ident) continue;
-    const int* __restrict__ rows = S.rows;
+    const gp<const int> rows = gload(S.rows);
+    const bool listed = S.rows != nullptr;
     const int row_base = S.row_base;
-    const int* __restrict__ a_rp = S.a_rp;
-    const int* __restrict__ c_rp = S.c_rp;
-    const long long* __restrict__ soff = S.soff;
-    unsigned short* __restrict__ map = S.map;
+    const gp<const int> a_rp = gload(S.a_rp);
+    const gp<const int> c_rp = gload(S.c_rp);
+    const gp<const long long> soff = gload(S.soff);
+    const gp<unsigned short> map = gstore(S.map);
+    const gp<float> c_val = gstore(S.c_val);
     const long long total = S.total;
     const int c_nnz = S.c_nnz;
     const int k = item - S.item0;
-    int i = S.item_first[k];
-    const int i_end = S.item_first[k + 1];
+    int i = gload(S.item_first)[k];
+    const int i_end = gload(S.item_first)[k + 1];
     while (i < i_end) {
       // One lane per row of the sub-batch.
       const int ii = i + lane;
@@ -145,14 +218,14 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
       int row = 0, cs = 0, n = 0, as = 0, ae = 0;
       long long rq = 0;
       if (rv) {
-        row = rows ? rows[ii] : row_base + ii;
+        row = listed ? rows[ii] : row_base + ii;
         cs = c_rp[row];
         n = c_rp[row + 1] - cs;
         as = a_rp[row];
         ae = a_rp[row + 1];
         rq = soff[ii];
       }
-      const int incl_n = group_incl_scan<kWave>(n);
+      const int incl_n = wave_incl_sum(n);
       int R = ident ? min(kWave, i_end - i) : __popcll(__ballot(rv && incl_n <= capw));
       R = __builtin_amdgcn_readfirstlane(R);
       if (R == 0) {
@@ -169,9 +242,9 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
       const int abase = incl_n - n;  // accumulator offset of the row (rows < R)
       const int totn = ident ? 0 : __shfl(incl_n, R - 1);
       const int ne = ae - as;
-      const int incl_e = group_incl_scan<kWave>(ne);
+      const int incl_e = wave_incl_sum(ne);
       const int ebase = incl_e - ne;
-      const int E = __shfl(incl_e, kWave - 1);
+      const int E = wave_last(incl_e);
       if (!ident) {
         // Accumulators cleared (numeric) or the rows' columns staged (build).
         int rc = 0;
@@ -181,9 +254,9 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
             continue;
           }
           const int r = wave_owner(mark, rl && n > 0, abase, j0, rc);
-          rc = __shfl(r, kWave - 1);
+          rc = wave_last(r);
           const int src = __shfl(cs, r) + j0 + lane - __shfl(abase, r);
-          if (j0 + lane < totn) cols[j0 + lane] = S.c_ci[src];
+          if (j0 + lane < totn) cols[j0 + lane] = gload(S.c_ci)[src];
         }
         wave_lds_order();
       }
@@ -192,7 +265,7 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
       for (int c0 = 0; c0 < E; c0 += kWave) {
         // One lane per A entry of the chunk.
         const int er = wave_owner(mark, rl && ne > 0, ebase, c0, ec);
-        ec = __shfl(er, kWave - 1);
+        ec = wave_last(er);
         const int eidx = c0 + lane;
         const bool ev = eidx < E;
         const int ap = __shfl(as, er) + eidx - __shfl(ebase, er);
@@ -208,10 +281,10 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
           bs = B.rp[kk];
           len = B.rp[kk + 1] - bs;
         }
-        const int incl_p = group_incl_scan<kWave>(len);
+        const int incl_p = wave_incl_sum(len);
         const int excl_p = incl_p - len;
         const int boff = bs - excl_p;  // B position of the entry's product at chunk offset t: boff + t
-        const int T = __shfl(incl_p, kWave - 1);
+        const int T = wave_last(incl_p);
         if (T > 0) {
           const long long w0 = qc >> 6, w1 = (qc + T - 1) >> 6;
           int pc = 0;
@@ -220,7 +293,7 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
                             int& p_cs, long long& p_rq) {
             const int off = (int)((w << 6) - qc);
             const int e = wave_owner(mark, len > 0, excl_p, off, pc);
-            pc = __shfl(e, kWave - 1);
+            pc = wave_last(e);
             const int t = off + lane;
             pv = t >= 0 && t < T;
             bi = __shfl(boff, e) + t;
@@ -246,45 +319,73 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
                 else map[qq] = (unsigned short)(pos | (qq == p_rq ? kRowMark : 0u));
               }
             }
-          } else if (ident) {
-            for (long long w = w0; w <= w1; ++w) {
-              bool pv;
-              int bi, p_acc, p_n, p_cs;
-              float av;
-              long long p_rq;
-              locate(w, pv, bi, av, p_acc, p_n, p_cs, p_rq);
-              const long long qq = (w << 6) + lane;
-              const long long dst = (long long)p_cs + (qq - p_rq);
-              if (pv && (dst < 0 || dst >= c_nnz)) bad |= kStreamBadValue;
-              else if (pv) S.c_val[dst] = acc_step(0.0f, av, B.val[bi]);
-            }
           } else {
-            // The next window's map entry and B value are in flight while
-            // this window is applied.
-            bool pv0, pv1;
-            int bi, acc0, acc1, p_n, p_cs;
-            float av0, av1, bv0, bv1;
-            unsigned me0, me1;
-            long long p_rq;
-            locate(w0, pv0, bi, av0, acc0, p_n, p_cs, p_rq);
-            me0 = pv0 ? map[(w0 << 6) + lane] : 0u;
-            bv0 = pv0 ? B.val[bi] : 0.0f;
-            for (long long w = w0; w <= w1; ++w) {
-              if (w < w1) {
-                locate(w + 1, pv1, bi, av1, acc1, p_n, p_cs, p_rq);
-                me1 = pv1 ? map[((w + 1) << 6) + lane] : 0u;
-                bv1 = pv1 ? B.val[bi] : 0.0f;
+            // Windows k = 0.. of the chunk: window k starts `lead` products
+            // before chunk offset 64 k. Each entry's operands come from the
+            // entry table with one LDS read; for map segments the next
+            // window's map entry and B value are in flight while this window
+            // is applied.
+            const long long qa = qc & ~63LL;
+            const int lead = (int)(qc - qa);
+            const int nwin = (lead + T + 63) >> 6;
+            // Loads of a whole window are issued unconditionally (the map has
+            // 128 entries of slack past the stream; lanes without a product
+            // read B position 0) so no branch separates them from their use.
+            const gp<const unsigned short> mc = ident ? nullptr : (gp<const unsigned short>)(map + qa);
+            // Identity rows: C position of the product at chunk offset t is
+            // base + t, base = the row's C start minus the chunk offset of
+            // the row's first product.
+            const int e_base = ident ? e_cs - (int)(e_rq - qc) : e_acc;
+            etab[lane] = make_int4(boff, __float_as_int(a), e_base, 0);
+            wave_lds_order();
+            auto step = [&](int k, bool& pv, int& t, int& bi, float& av, int& base) {
+              const int off = (k << 6) - lead;
+              const int e = wave_owner(mark, len > 0, excl_p, off, pc);
+              pc = wave_last(e);
+              t = off + lane;
+              pv = t >= 0 && t < T;
+              const int4 et = etab[e];
+              bi = et.x + t;
+              av = __int_as_float(et.y);
+              base = et.z;
+            };
+            if (ident) {
+              for (int k = 0; k < nwin; ++k) {
+                bool pv;
+                int t, bi, base;
+                float av;
+                step(k, pv, t, bi, av, base);
+                const int dst = base + t;
+                if (pv && (dst < 0 || dst >= c_nnz)) bad |= kStreamBadValue;
+                else if (pv) c_val[dst] = acc_step(0.0f, av, B.val[bi]);
               }
-              const int at = acc0 + (int)(me0 & kPosMask);
-              const bool ok = !pv0 || (at >= 0 && at < totn);
-              if (!ok) bad |= kStreamBadLds;
-              stream_apply(vals, pv0 && ok, at, me0 >> kOrderShift, av0, bv0);
-              pv0 = pv1;
-              acc0 = acc1;
-              av0 = av1;
-              bv0 = bv1;
-              me0 = me1;
+            } else {
+              bool pv0, pv1 = false;
+              int t, bi, acc0, acc1 = 0;
+              float av0, av1 = 0.0f, bv0, bv1 = 0.0f;
+              unsigned me0, me1 = 0u;
+              step(0, pv0, t, bi, av0, acc0);
+              me0 = mc[lane];
+              bv0 = B.val[pv0 ? bi : 0];
+              for (int k = 0; k < nwin; ++k) {
+                // Unconditional, also past the last window (no products
+                // there; its map entries lie in the slack), so the compiler
+                // keeps these loads in flight across the adds below.
+                step(k + 1, pv1, t, bi, av1, acc1);
+                me1 = mc[((k + 1) << 6) + lane];
+                bv1 = B.val[pv1 ? bi : 0];
+                const int at = acc0 + (int)(me0 & kPosMask);
+                const bool ok = !pv0 || (at >= 0 && at < totn);
+                if (!ok) bad |= kStreamBadLds;
+                stream_apply(vals, pv0 && ok, at, me0 >> kOrderShift, av0, bv0);
+                pv0 = pv1;
+                acc0 = acc1;
+                av0 = av1;
+                bv0 = bv1;
+                me0 = me1;
+              }
             }
+            wave_lds_order();  // the entry table is rewritten by the next chunk
           }
         }
         qc += T;
@@ -295,11 +396,11 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
         int rc = 0;
         for (int j0 = 0; j0 < totn; j0 += kWave) {
           const int r = wave_owner(mark, rl && n > 0, abase, j0, rc);
-          rc = __shfl(r, kWave - 1);
+          rc = wave_last(r);
           const int dst = __shfl(cs, r) + j0 + lane - __shfl(abase, r);
           if (j0 + lane < totn) {
             if (dst < 0 || dst >= c_nnz) bad |= kStreamBadValue;
-            else S.c_val[dst] = vals[j0 + lane];
+            else c_val[dst] = vals[j0 + lane];
           }
         }
         wave_lds_order();
@@ -315,7 +416,7 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
   if (lane == 0) {
     const unsigned d = atomicAdd(&q->done, 1u);
     if (d == gridDim.x - 1) {
-      atomicExch(&q->next, 0u);
+      for (int x = 0; x < kStreamQueues; ++x) atomicExch(&q->next[x], 0u);
       atomicExch(&q->done, 0u);
     }
   }
@@ -340,6 +441,10 @@ __global__ __launch_bounds__(kBlock) void k_stream_rounds(unsigned short* __rest
   const unsigned ord = before < (int)kLaneOrder ? (unsigned)before : kLaneOrder;
   if (v) map[qq] = (unsigned short)((ent & (kPosMask | kRowMark)) | (ord << kOrderShift));
 }
+
+// Accumulators (or columns) of capw positions, rounded to 16 bytes, the
+// marks and the entry table.
+int stream_lds_bytes(int capw) { return ((capw + 3) & ~3) * 4 + kWave * 4 + kWave * 16; }
 
 int stream_waves_per_cu(bool build, int lds) {
   int nb = 0;
@@ -380,8 +485,23 @@ int stream_grid(bool build, int capw, int items) {
     (void)hipGetLastError();
     cus = 256;
   }
-  const long long fit = (long long)stream_waves_per_cu(build, (capw + kWave) * 4) * cus;
+  const long long fit = (long long)stream_waves_per_cu(build, stream_lds_bytes(capw)) * cus;
   return (int)(fit < items ? fit : items);
+}
+
+hipError_t launch_stream_mark(const int* rows, int row_base, int nrows, int* flags, hipStream_t s) {
+  if (nrows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_stream_mark, dim3((nrows + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rows,
+                     row_base, nrows, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_stream_compact(const int* flags, const int* pos, int M, int* list,
+                                 hipStream_t s) {
+  if (M <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_stream_compact, dim3((M + kBlock - 1) / kBlock), dim3(kBlock), 0, s, flags,
+                     pos, M, list);
+  return hipGetLastError();
 }
 
 hipError_t launch_stream(bool build, const StreamSeg* segs, int nseg, int items, const DevCsr& A,
@@ -389,7 +509,7 @@ hipError_t launch_stream(bool build, const StreamSeg* segs, int nseg, int items,
   if (items <= 0 || nseg <= 0) return hipSuccess;
   if (nseg > kStreamMaxSegs || capw < 0 || capw > kStreamMaxRow || grid <= 0)
     return hipErrorInvalidValue;
-  const int lds = (capw + kWave) * 4;
+  const int lds = stream_lds_bytes(capw);
   if (build)
     hipLaunchKernelGGL(k_stream<true>, dim3(grid), dim3(kWave), lds, s, segs, nseg, items, A, B,
                        q, capw);
