@@ -499,8 +499,8 @@ rvsp_status_t build_stream(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream
     RVSP_TRY(ensure(part.items, ((size_t)nitems + 1) * 4));
     RVSP_HIP_TRY(launch_stream_items((const long long*)part.soff.p, pk.nrows, nitems,
                                      (int*)part.items.p, s));
-    // 128 entries of slack: the numeric pass loads whole windows, one ahead.
-    if (!pk.ident && totals[j] > 0) RVSP_TRY(ensure(part.map, (size_t)totals[j] * 2 + 256));
+    // 256 entries of slack: the numeric pass loads whole windows ahead.
+    if (!pk.ident && totals[j] > 0) RVSP_TRY(ensure(part.map, (size_t)totals[j] * 2 + 512));
     StreamSeg& sg = bs.segs[j];
     sg.rows = pk.row_base >= 0 ? nullptr : pk.rows;
     sg.row_base = pk.row_base >= 0 ? pk.row_base : 0;
@@ -534,8 +534,8 @@ rvsp_status_t build_stream(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream
   RVSP_HIP_TRY(hipStreamSynchronize(s));
   if (bad) return RVSP_ERROR_DEVICE;  // a map that does not match the structure
   bs.stream_items = item0;
-  bs.stream_capw = capw;
-  bs.stream_grid = stream_grid(false, capw, item0);
+  bs.stream_capw = std::min(capw, kStreamNumericRow);
+  bs.stream_grid = stream_grid(false, bs.stream_capw, item0);
   for (const Pick& pk : picks) bs.groups[pk.group].inner->stream_bins |= pk.bins;
   bs.stream_failed = false;
   bs.stream_ready = true;

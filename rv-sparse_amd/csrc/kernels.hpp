@@ -301,6 +301,9 @@ constexpr unsigned kStreamBadLds = 4u;     // accumulator outside the sub-batch
 constexpr unsigned kStreamBadColumn = 8u;  // product column not in its C row
 constexpr int kStreamItem = 8192;
 constexpr int kStreamMaxRow = 2048;
+// Accumulators per wave of the numeric pass; longer rows are walked once
+// per range of this many positions.
+constexpr int kStreamNumericRow = 1024;
 constexpr int kStreamMaxSegs = 256;
 constexpr int kStreamPosBits = 11;
 // w[i] = ops[row of list position i].

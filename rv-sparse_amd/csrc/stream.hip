@@ -138,6 +138,7 @@ __device__ __forceinline__ void stream_apply(float* vals, bool pv, int at, unsig
     if (pv) vals[at] = acc_step(vals[at], a, b);
     return;
   }
+#pragma unroll 1
   for (unsigned r = 0; r < kLaneOrder; ++r) {
     if (__ballot(pv && ord >= r) == 0) return;
     if (pv && ord == r) vals[at] = acc_step(vals[at], a, b);
@@ -226,13 +227,20 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
         rq = soff[ii];
       }
       const int incl_n = wave_incl_sum(n);
+      // Rows of the sub-batch: as many as the accumulators hold. A row longer
+      // than the accumulators (numeric pass only; the build gets room for
+      // every row) is a sub-batch of its own, walked once per range of capw
+      // positions, each walk applying only the products of its range.
       int R = ident ? min(kWave, i_end - i) : __popcll(__ballot(rv && incl_n <= capw));
       R = __builtin_amdgcn_readfirstlane(R);
+      int npass = 1;
       if (R == 0) {
-        // A row longer than the accumulators (the host never plans one):
-        // skipped rather than overrun.
-        ++i;
-        continue;
+        if (BUILD) {  // never planned: skipped rather than overrun
+          ++i;
+          continue;
+        }
+        R = 1;
+        npass = (__builtin_amdgcn_readfirstlane(n) + capw - 1) / capw;
       }
       const bool rl = lane < R;
       if (!rl) {
@@ -245,166 +253,189 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
       const int incl_e = wave_incl_sum(ne);
       const int ebase = incl_e - ne;
       const int E = wave_last(incl_e);
-      if (!ident) {
-        // Accumulators cleared (numeric) or the rows' columns staged (build).
-        int rc = 0;
-        for (int j0 = 0; j0 < totn; j0 += kWave) {
-          if (!BUILD) {
-            if (j0 + lane < totn) vals[j0 + lane] = 0.0f;
-            continue;
-          }
-          const int r = wave_owner(mark, rl && n > 0, abase, j0, rc);
-          rc = wave_last(r);
-          const int src = __shfl(cs, r) + j0 + lane - __shfl(abase, r);
-          if (j0 + lane < totn) cols[j0 + lane] = gload(S.c_ci)[src];
-        }
-        wave_lds_order();
-      }
-      long long qc = __shfl(rq, 0);  // stream index of the chunk's first product
-      int ec = 0;
-      for (int c0 = 0; c0 < E; c0 += kWave) {
-        // One lane per A entry of the chunk.
-        const int er = wave_owner(mark, rl && ne > 0, ebase, c0, ec);
-        ec = wave_last(er);
-        const int eidx = c0 + lane;
-        const bool ev = eidx < E;
-        const int ap = __shfl(as, er) + eidx - __shfl(ebase, er);
-        const int e_acc = __shfl(abase, er);
-        const int e_n = __shfl(n, er);
-        const int e_cs = __shfl(cs, er);
-        const long long e_rq = __shfl(rq, er);
-        int bs = 0, len = 0;
-        float a = 0.0f;
-        if (ev) {
-          const int kk = A.ci[ap];
-          a = A.val[ap];
-          bs = B.rp[kk];
-          len = B.rp[kk + 1] - bs;
-        }
-        const int incl_p = wave_incl_sum(len);
-        const int excl_p = incl_p - len;
-        const int boff = bs - excl_p;  // B position of the entry's product at chunk offset t: boff + t
-        const int T = wave_last(incl_p);
-        if (T > 0) {
-          const long long w0 = qc >> 6, w1 = (qc + T - 1) >> 6;
-          int pc = 0;
-          // Entry, B position and operands of every lane of window w.
-          auto locate = [&](long long w, bool& pv, int& bi, float& av, int& p_acc, int& p_n,
-                            int& p_cs, long long& p_rq) {
-            const int off = (int)((w << 6) - qc);
-            const int e = wave_owner(mark, len > 0, excl_p, off, pc);
-            pc = wave_last(e);
-            const int t = off + lane;
-            pv = t >= 0 && t < T;
-            bi = __shfl(boff, e) + t;
-            av = __shfl(a, e);
-            p_acc = __shfl(e_acc, e);
-            p_n = __shfl(e_n, e);
-            p_cs = __shfl(e_cs, e);
-            p_rq = __shfl(e_rq, e);
-          };
-          if (BUILD) {
-            for (long long w = w0; w <= w1; ++w) {
-              bool pv;
-              int bi, p_acc, p_n, p_cs;
-              float av;
-              long long p_rq;
-              locate(w, pv, bi, av, p_acc, p_n, p_cs, p_rq);
-              const long long qq = (w << 6) + lane;
-              if (pv) {
-                const int col = B.ci[bi];
-                const unsigned pos = (unsigned)lds_find(cols + p_acc, p_n, col);
-                if (cols[p_acc + pos] != col || pos > kPosMask) bad |= kStreamBadColumn;
-                if (qq < 0 || qq >= total) bad |= kStreamBadMap;
-                else map[qq] = (unsigned short)(pos | (qq == p_rq ? kRowMark : 0u));
-              }
+      for (int pass = 0; pass < npass; ++pass) {
+        const int p0 = pass * capw;  // first position of this walk
+        const int pn = npass > 1 ? min(capw, totn - p0) : totn;  // accumulators in use
+        if (!ident) {
+          // Accumulators cleared (numeric) or the rows' columns staged (build).
+          int rc = 0;
+          for (int j0 = 0; j0 < pn; j0 += kWave) {
+            if (!BUILD) {
+              if (j0 + lane < pn) vals[j0 + lane] = 0.0f;
+              continue;
             }
-          } else {
-            // Windows k = 0.. of the chunk: window k starts `lead` products
-            // before chunk offset 64 k. Each entry's operands come from the
-            // entry table with one LDS read; for map segments the next
-            // window's map entry and B value are in flight while this window
-            // is applied.
-            const long long qa = qc & ~63LL;
-            const int lead = (int)(qc - qa);
-            const int nwin = (lead + T + 63) >> 6;
-            // Loads of a whole window are issued unconditionally (the map has
-            // 128 entries of slack past the stream; lanes without a product
-            // read B position 0) so no branch separates them from their use.
-            const gp<const unsigned short> mc = ident ? nullptr : (gp<const unsigned short>)(map + qa);
-            // Identity rows: C position of the product at chunk offset t is
-            // base + t, base = the row's C start minus the chunk offset of
-            // the row's first product.
-            const int e_base = ident ? e_cs - (int)(e_rq - qc) : e_acc;
-            etab[lane] = make_int4(boff, __float_as_int(a), e_base, 0);
-            wave_lds_order();
-            auto step = [&](int k, bool& pv, int& t, int& bi, float& av, int& base) {
-              const int off = (k << 6) - lead;
+            const int r = wave_owner(mark, rl && n > 0, abase, j0, rc);
+            rc = wave_last(r);
+            const int src = __shfl(cs, r) + j0 + lane - __shfl(abase, r);
+            if (j0 + lane < totn) cols[j0 + lane] = gload(S.c_ci)[src];
+          }
+          wave_lds_order();
+        }
+        long long qc = __shfl(rq, 0);  // stream index of the chunk's first product
+        int ec = 0;
+        for (int c0 = 0; c0 < E; c0 += kWave) {
+          // One lane per A entry of the chunk.
+          const int er = wave_owner(mark, rl && ne > 0, ebase, c0, ec);
+          ec = wave_last(er);
+          const int eidx = c0 + lane;
+          const bool ev = eidx < E;
+          const int ap = __shfl(as, er) + eidx - __shfl(ebase, er);
+          const int e_acc = __shfl(abase, er);
+          const int e_n = __shfl(n, er);
+          const int e_cs = __shfl(cs, er);
+          const long long e_rq = __shfl(rq, er);
+          int bs = 0, len = 0;
+          float a = 0.0f;
+          if (ev) {
+            const int kk = A.ci[ap];
+            a = A.val[ap];
+            bs = B.rp[kk];
+            len = B.rp[kk + 1] - bs;
+          }
+          const int incl_p = wave_incl_sum(len);
+          const int excl_p = incl_p - len;
+          const int boff = bs - excl_p;  // B position of the entry's product at chunk offset t: boff + t
+          const int T = wave_last(incl_p);
+          if (T > 0) {
+            const long long w0 = qc >> 6, w1 = (qc + T - 1) >> 6;
+            int pc = 0;
+            // Entry, B position and operands of every lane of window w.
+            auto locate = [&](long long w, bool& pv, int& bi, float& av, int& p_acc, int& p_n,
+                              int& p_cs, long long& p_rq) {
+              const int off = (int)((w << 6) - qc);
               const int e = wave_owner(mark, len > 0, excl_p, off, pc);
               pc = wave_last(e);
-              t = off + lane;
+              const int t = off + lane;
               pv = t >= 0 && t < T;
-              const int4 et = etab[e];
-              bi = et.x + t;
-              av = __int_as_float(et.y);
-              base = et.z;
+              bi = __shfl(boff, e) + t;
+              av = __shfl(a, e);
+              p_acc = __shfl(e_acc, e);
+              p_n = __shfl(e_n, e);
+              p_cs = __shfl(e_cs, e);
+              p_rq = __shfl(e_rq, e);
             };
-            if (ident) {
-              for (int k = 0; k < nwin; ++k) {
+            if (BUILD) {
+              for (long long w = w0; w <= w1; ++w) {
                 bool pv;
-                int t, bi, base;
+                int bi, p_acc, p_n, p_cs;
                 float av;
-                step(k, pv, t, bi, av, base);
-                const int dst = base + t;
-                if (pv && (dst < 0 || dst >= c_nnz)) bad |= kStreamBadValue;
-                else if (pv) c_val[dst] = acc_step(0.0f, av, B.val[bi]);
+                long long p_rq;
+                locate(w, pv, bi, av, p_acc, p_n, p_cs, p_rq);
+                const long long qq = (w << 6) + lane;
+                if (pv) {
+                  const int col = B.ci[bi];
+                  const unsigned pos = (unsigned)lds_find(cols + p_acc, p_n, col);
+                  if (cols[p_acc + pos] != col || pos > kPosMask) bad |= kStreamBadColumn;
+                  if (qq < 0 || qq >= total) bad |= kStreamBadMap;
+                  else map[qq] = (unsigned short)(pos | (qq == p_rq ? kRowMark : 0u));
+                }
               }
             } else {
-              bool pv0, pv1 = false;
-              int t, bi, acc0, acc1 = 0;
-              float av0, av1 = 0.0f, bv0, bv1 = 0.0f;
-              unsigned me0, me1 = 0u;
-              step(0, pv0, t, bi, av0, acc0);
-              me0 = mc[lane];
-              bv0 = B.val[pv0 ? bi : 0];
-              for (int k = 0; k < nwin; ++k) {
-                // Unconditional, also past the last window (no products
-                // there; its map entries lie in the slack), so the compiler
-                // keeps these loads in flight across the adds below.
-                step(k + 1, pv1, t, bi, av1, acc1);
-                me1 = mc[((k + 1) << 6) + lane];
-                bv1 = B.val[pv1 ? bi : 0];
-                const int at = acc0 + (int)(me0 & kPosMask);
-                const bool ok = !pv0 || (at >= 0 && at < totn);
-                if (!ok) bad |= kStreamBadLds;
-                stream_apply(vals, pv0 && ok, at, me0 >> kOrderShift, av0, bv0);
-                pv0 = pv1;
-                acc0 = acc1;
-                av0 = av1;
-                bv0 = bv1;
-                me0 = me1;
+              // Windows k = 0.. of the chunk: window k starts `lead` products
+              // before chunk offset 64 k. Each entry's operands come from the
+              // entry table with one LDS read; for map segments the next
+              // window's map entry and B value are in flight while this window
+              // is applied.
+              const long long qa = qc & ~63LL;
+              const int lead = (int)(qc - qa);
+              const int nwin = (lead + T + 63) >> 6;
+              // Map reads cover whole windows, up to three past the chunk's
+              // last (the map has 256 entries of slack past the stream); lanes
+              // without a product read B position 0.
+              const gp<const unsigned short> mc = ident ? nullptr : (gp<const unsigned short>)(map + qa);
+              // Identity rows: C position of the product at chunk offset t is
+              // base + t, base = the row's C start minus the chunk offset of
+              // the row's first product.
+              const int e_base = ident ? e_cs - (int)(e_rq - qc) : e_acc;
+              etab[lane] = make_int4(boff, __float_as_int(a), e_base, 0);
+              wave_lds_order();
+              auto step = [&](int k, bool& pv, int& t, int& bi, float& av, int& base) {
+                const int off = (k << 6) - lead;
+                const int e = wave_owner(mark, len > 0, excl_p, off, pc);
+                pc = wave_last(e);
+                t = off + lane;
+                pv = t >= 0 && t < T;
+                const int4 et = etab[e];
+                bi = et.x + t;
+                av = __int_as_float(et.y);
+                base = et.z;
+              };
+              if (ident) {
+                for (int k = 0; k < nwin; ++k) {
+                  bool pv;
+                  int t, bi, base;
+                  float av;
+                  step(k, pv, t, bi, av, base);
+                  const int dst = base + t;
+                  if (pv && (dst < 0 || dst >= c_nnz)) bad |= kStreamBadValue;
+                  else if (pv) c_val[dst] = acc_step(0.0f, av, B.val[bi]);
+                }
+              } else {
+                // Windows k + 1 and k + 2 are in flight while window k is
+                // applied. Their loads are unconditional, also past the last
+                // window (no products there; map entries in the slack), so no
+                // branch makes the compiler wait for them early.
+                struct Win {
+                  bool pv;
+                  int acc;
+                  float av, bv;
+                  unsigned me;
+                };
+                auto issue = [&](int k) {
+                  Win w;
+                  int t, bi;
+                  step(k, w.pv, t, bi, w.av, w.acc);
+                  w.me = mc[(k << 6) + lane];
+                  w.bv = B.val[w.pv ? bi : 0];
+                  return w;
+                };
+                auto apply = [&](const Win& w) {
+                  const int rel = (int)(w.me & kPosMask) - p0;
+                  const int at = w.acc + rel;
+                  const bool use = w.pv && rel >= 0 && at < pn;
+                  if (npass == 1 && w.pv && !use) bad |= kStreamBadLds;
+                  stream_apply(vals, use, at, w.me >> kOrderShift, w.av, w.bv);
+                };
+                // Unrolled by three so the windows' registers change roles
+                // without copies (a copy of a load's destination would wait
+                // for the load).
+                Win wa = issue(0);
+                Win wb = issue(1);
+                Win wc;
+                for (int k = 0;; k += 3) {
+                  wc = issue(k + 2);
+                  apply(wa);
+                  if (k + 1 >= nwin) break;
+                  wa = issue(k + 3);
+                  apply(wb);
+                  if (k + 2 >= nwin) break;
+                  wb = issue(k + 4);
+                  apply(wc);
+                  if (k + 3 >= nwin) break;
+                }
               }
+              wave_lds_order();  // the entry table is rewritten by the next chunk
             }
-            wave_lds_order();  // the entry table is rewritten by the next chunk
           }
+          qc += T;
         }
-        qc += T;
-      }
-      if (!BUILD && !ident) {
-        // Accumulators to C, row by row in order (coalesced runs).
-        wave_lds_order();
-        int rc = 0;
-        for (int j0 = 0; j0 < totn; j0 += kWave) {
-          const int r = wave_owner(mark, rl && n > 0, abase, j0, rc);
-          rc = wave_last(r);
-          const int dst = __shfl(cs, r) + j0 + lane - __shfl(abase, r);
-          if (j0 + lane < totn) {
-            if (dst < 0 || dst >= c_nnz) bad |= kStreamBadValue;
-            else c_val[dst] = vals[j0 + lane];
+        if (!BUILD && !ident) {
+          // Accumulators to C, row by row in order (coalesced runs); a walk
+          // of one long row writes its range.
+          wave_lds_order();
+          int rc = 0;
+          for (int j0 = 0; j0 < pn; j0 += kWave) {
+            const int r = wave_owner(mark, rl && n > 0, abase, j0, rc);
+            rc = wave_last(r);
+            const int dst = __shfl(cs, r) + p0 + j0 + lane - __shfl(abase, r);
+            if (j0 + lane < pn) {
+              if (dst < 0 || dst >= c_nnz) bad |= kStreamBadValue;
+              else c_val[dst] = vals[j0 + lane];
+            }
           }
+          wave_lds_order();
         }
-        wave_lds_order();
-      }
+      }  // walks
       i += R;
     }
   }
