@@ -121,9 +121,14 @@ struct BatchState {
   };
   std::vector<StreamPart> parts;
   std::vector<StreamSeg> segs;
+  std::vector<int> seg_group;  // group of every segment
   DevBuf d_segs, d_queue, s_weights, s_tiles, s_totals;
-  int stream_items = 0, stream_capw = 0, stream_grid = 0;
-  bool stream_ready = false, stream_failed = false;
+  int stream_items = 0;
+  int build_capw = 0, map_capw = 0, search_capw = 0;  // per pass kind
+  int map_grid = 0, search_grid = 0;
+  bool stream_ready = false, stream_failed = false;  // segments prepared
+  bool map_ready = false, map_failed = false;        // map built (or refused)
+  bool segs_stale = true;  // output pointers changed since the last upload
   int graph_launches = 0;
   int repeats = 0;  // consecutive numeric-only computes with the same arrays
   // Eager computes run the groups on kLanes streams at once (fork and join
@@ -189,10 +194,14 @@ struct BatchState {
     }
     parts.clear();
     segs.clear();
+    seg_group.clear();
     for (BatchGroup& g : groups)
       if (g.inner) g.inner->stream_bins = 0;
     stream_ready = false;
     stream_failed = false;
+    map_ready = false;
+    map_failed = false;
+    segs_stale = true;
     stream_items = 0;
   }
   ~BatchState() {
@@ -354,21 +363,39 @@ void build_output_tables(BatchState& bs, const rvsp_csr_matrix_t* C) {
   }
 }
 
-// Sets up the persistent row-stream numeric pass after the first full
-// compute: every group's numeric bin of rows fed by one B row (identity
-// positions) and every bin of rows up to kStreamMaxRow long becomes a
-// segment; per segment the list-order product offsets, the queue-item
-// starts, and for non-identity bins the position map (built on the device
-// from C's columns, then its order bits). Synchronous. On success the
-// groups skip those bins; on failure (memory, too many segments) nothing
-// changes and the per-bin launches stay.
-rvsp_status_t build_stream(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream_t s) {
+DevCsr stacked_a(const BatchState& bs) {
+  DevCsr A;
+  A.rows = (int)bs.a_row_off.back();
+  A.cols = (int)bs.b_row_off.back();
+  A.nnz = (int)bs.a_nnz_off.back();
+  A.ci = (const int*)bs.a_ci.p;
+  A.val = (const float*)bs.a_val.p;
+  return A;
+}
+
+DevCsr stacked_b(const BatchState& bs) {
+  DevCsr B;
+  B.rows = (int)bs.b_row_off.back();
+  B.cols = bs.b_cols;
+  B.nnz = (int)bs.b_nnz_off.back();
+  B.rp = (const int*)bs.b_rp.p;
+  B.ci = (const int*)bs.b_ci.p;
+  B.val = (const float*)bs.b_val.p;
+  return B;
+}
+
+// Prepares the persistent row-stream numeric pass once the groups are
+// estimated: per group up to two segments, the rows fed by one B row (bin 1,
+// positions need no map) and every other row of a numeric bin whose rows are
+// at most kStreamMaxRow long, in ascending row order (one pair's rows next to
+// each other, so the waves walking them read the same B). Per segment the
+// row list, the list-order product offsets and the queue-item starts.
+// Synchronous. On success the groups skip those bins in every compute and the
+// batch runs them in one launch (search pass until the map is built); on
+// failure nothing changes and the per-bin launches stay.
+rvsp_status_t prepare_stream(BatchState& bs, hipStream_t s) {
   bs.drop_stream();
   bs.stream_failed = true;  // until every step below has succeeded
-  // Per group up to two segments: the rows fed by one B row (bin 1, no map),
-  // and every other row of a bin whose rows are at most kStreamMaxRow long,
-  // in ascending row order (one pair's rows next to each other, so the
-  // waves walking them read the same B).
   struct Pick {
     int group, nrows, row_base;
     const int* rows;
@@ -381,7 +408,7 @@ rvsp_status_t build_stream(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream
     const rvsp_spgemm_descr* in = bs.groups[gi].inner;
     const PhasePlan& p = in->numeric;
     const int M = (int)(bs.groups[gi].r1 - bs.groups[gi].r0);
-    if (bin_rows(p, 1) > 0) {
+    if (p.table.rule.nbins > 1 && bin_rows(p, 1) > 0) {
       Pick pk;
       pk.group = (int)gi;
       pk.nrows = bin_rows(p, 1);
@@ -413,19 +440,19 @@ rvsp_status_t build_stream(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream
     }
   }
   if (picks.empty()) return RVSP_SUCCESS;
+  const int ns = (int)picks.size();
+  bs.parts.resize(ns);
+  RVSP_TRY(ensure(bs.s_tiles, scan_tile_count(std::max(max_rows, max_group_rows)) * 8));
+  RVSP_TRY(ensure(bs.s_totals, (size_t)ns * 8));
   // Row lists of the merged segments: rows of their bins, in row order.
-  bs.parts.resize(picks.size());
   if (max_group_rows > 0) {
     DevBuf flags, pos;
     RVSP_TRY(ensure(flags, (size_t)max_group_rows * 4));
     RVSP_TRY(ensure(pos, ((size_t)max_group_rows + 1) * 4));
-    RVSP_TRY(ensure(bs.s_tiles, scan_tile_count(std::max(max_rows, max_group_rows)) * 8));
-    RVSP_TRY(ensure(bs.s_totals, picks.size() * 8));
-    for (size_t j = 0; j < picks.size(); ++j) {
+    for (int j = 0; j < ns; ++j) {
       Pick& pk = picks[j];
       if (pk.ident) continue;
-      const rvsp_spgemm_descr* in = bs.groups[pk.group].inner;
-      const PhasePlan& p = in->numeric;
+      const PhasePlan& p = bs.groups[pk.group].inner->numeric;
       const int M = (int)(bs.groups[pk.group].r1 - bs.groups[pk.group].r0);
       RVSP_HIP_TRY(hipMemsetAsync(flags.p, 0, (size_t)M * 4, s));
       for (int b = 2; b < p.table.rule.nbins; ++b)
@@ -444,10 +471,8 @@ rvsp_status_t build_stream(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream
     release_idle(flags);
     release_idle(pos);
   }
-  const int ns = (int)picks.size();
+  // Products per list position, their scan (the stream offsets) and totals.
   RVSP_TRY(ensure(bs.s_weights, (size_t)max_rows * 8));
-  RVSP_TRY(ensure(bs.s_tiles, scan_tile_count(max_rows) * 8));
-  RVSP_TRY(ensure(bs.s_totals, (size_t)ns * 8));
   for (int j = 0; j < ns; ++j) {
     const Pick& pk = picks[j];
     RVSP_TRY(ensure(bs.parts[j].soff, ((size_t)pk.nrows + 1) * 8));
@@ -462,34 +487,19 @@ rvsp_status_t build_stream(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream
   RVSP_HIP_TRY(hipMemcpyAsync(totals.data(), bs.s_totals.p, (size_t)ns * 8,
                               hipMemcpyDeviceToHost, s));
   RVSP_HIP_TRY(hipStreamSynchronize(s));
-  size_t map_bytes = 0;
   long long items = 0;
   int capw = 0;
   for (int j = 0; j < ns; ++j) {
     if (!picks[j].ident) {
-      map_bytes += (size_t)totals[j] * 2;
       const PhasePlan& p = bs.groups[picks[j].group].inner->numeric;
       for (int b = 2; b < p.table.rule.nbins; ++b)
         if ((picks[j].bins >> b) & 1u) capw = std::max(capw, p.table.rule.bound[b]);
     }
     items += (totals[j] + kStreamItem - 1) / kStreamItem;
   }
-  capw = std::min(capw, kStreamMaxRow);
-  const size_t budget = std::min<size_t>((size_t)32 << 30, device_free_bytes() / 3);
-  if (map_bytes > budget || items > INT_MAX / 2) return RVSP_SUCCESS;
-  DevCsr A, B;
-  A.rows = (int)bs.a_row_off.back();
-  A.cols = (int)bs.b_row_off.back();
-  A.nnz = (int)bs.a_nnz_off.back();
-  A.ci = (const int*)bs.a_ci.p;
-  A.val = (const float*)bs.a_val.p;
-  B.rows = (int)bs.b_row_off.back();
-  B.cols = bs.b_cols;
-  B.nnz = (int)bs.b_nnz_off.back();
-  B.rp = (const int*)bs.b_rp.p;
-  B.ci = (const int*)bs.b_ci.p;
-  B.val = (const float*)bs.b_val.p;
+  if (items > INT_MAX / 2) return RVSP_SUCCESS;
   bs.segs.assign(ns, StreamSeg());
+  bs.seg_group.assign(ns, 0);
   int item0 = 0;
   for (int j = 0; j < ns; ++j) {
     const Pick& pk = picks[j];
@@ -499,46 +509,90 @@ rvsp_status_t build_stream(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream
     RVSP_TRY(ensure(part.items, ((size_t)nitems + 1) * 4));
     RVSP_HIP_TRY(launch_stream_items((const long long*)part.soff.p, pk.nrows, nitems,
                                      (int*)part.items.p, s));
-    // 256 entries of slack: the numeric pass loads whole windows ahead.
-    if (!pk.ident && totals[j] > 0) RVSP_TRY(ensure(part.map, (size_t)totals[j] * 2 + 512));
     StreamSeg& sg = bs.segs[j];
     sg.rows = pk.row_base >= 0 ? nullptr : pk.rows;
     sg.row_base = pk.row_base >= 0 ? pk.row_base : 0;
     sg.nrows = pk.nrows;
     sg.a_rp = (const int*)bs.a_rp.p + g.r0;
     sg.c_rp = (const int*)g.inner->c_rp.p;
-    sg.c_ci = g.in_place ? (const int*)C[g.p0].col_idx : (const int*)g.c_ci.p;
-    sg.c_val = g.in_place ? (float*)C[g.p0].values : (float*)g.c_val.p;
     sg.soff = (const long long*)part.soff.p;
     sg.item_first = (const int*)part.items.p;
-    sg.map = pk.ident ? nullptr : (unsigned short*)part.map.p;
     sg.total = totals[j];
     sg.c_nnz = g.c_nnz;
+    sg.ident = pk.ident ? 1 : 0;
     sg.item0 = item0;
     item0 += nitems;
+    bs.seg_group[j] = pk.group;
   }
   RVSP_TRY(ensure(bs.d_segs, (size_t)ns * sizeof(StreamSeg)));
-  RVSP_HIP_TRY(hipMemcpyAsync(bs.d_segs.p, bs.segs.data(), (size_t)ns * sizeof(StreamSeg),
-                              hipMemcpyHostToDevice, s));
   if (!bs.d_queue.p) RVSP_TRY(ensure(bs.d_queue, sizeof(StreamQueue)));
   RVSP_HIP_TRY(hipMemsetAsync(bs.d_queue.p, 0, sizeof(StreamQueue), s));
-  // Map entries (positions and row marks), then their order bits.
-  const int bgrid = stream_grid(true, capw, item0);
-  RVSP_HIP_TRY(launch_stream(true, (const StreamSeg*)bs.d_segs.p, ns, item0, A, B,
-                             (StreamQueue*)bs.d_queue.p, capw, bgrid, s));
-  for (int j = 0; j < ns; ++j)
-    if (!picks[j].ident) RVSP_HIP_TRY(launch_stream_rounds(bs.segs[j].map, totals[j], s));
+  RVSP_HIP_TRY(hipStreamSynchronize(s));
+  bs.stream_items = item0;
+  bs.build_capw = std::min(capw, kStreamMaxRow);
+  bs.map_capw = std::min(bs.build_capw, kStreamNumericRow);
+  bs.search_capw = std::min(bs.build_capw, kStreamSearchRow);
+  bs.map_grid = stream_grid(kStreamMapPass, bs.map_capw, item0);
+  bs.search_grid = stream_grid(kStreamSearchPass, bs.search_capw, item0);
+  for (const Pick& pk : picks) bs.groups[pk.group].inner->stream_bins |= pk.bins;
+  bs.segs_stale = true;
+  bs.stream_failed = false;
+  bs.stream_ready = true;
+  return RVSP_SUCCESS;
+}
+
+// Output arrays of the segments (they follow the caller's C arrays) into the
+// device table, before a launch that reads them.
+rvsp_status_t upload_segments(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream_t s) {
+  if (!bs.segs_stale) return RVSP_SUCCESS;
+  for (size_t j = 0; j < bs.segs.size(); ++j) {
+    const BatchGroup& g = bs.groups[bs.seg_group[j]];
+    StreamSeg& sg = bs.segs[j];
+    sg.c_ci = g.in_place ? (const int*)C[g.p0].col_idx : (const int*)g.c_ci.p;
+    sg.c_val = g.in_place ? (float*)C[g.p0].values : (float*)g.c_val.p;
+    sg.map = sg.ident || !bs.map_ready ? nullptr : (unsigned short*)bs.parts[j].map.p;
+  }
+  RVSP_HIP_TRY(hipMemcpyAsync(bs.d_segs.p, bs.segs.data(), bs.segs.size() * sizeof(StreamSeg),
+                              hipMemcpyHostToDevice, s));
+  bs.segs_stale = false;
+  return RVSP_SUCCESS;
+}
+
+// Builds the position map of every non-identity segment (positions and row
+// marks by the build pass, then their order bits) for the numeric-only
+// computes. Synchronous. Over the memory budget, or if the build refused a
+// write, the map stays off and those computes keep the search pass.
+rvsp_status_t build_stream_map(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream_t s) {
+  bs.map_failed = true;
+  size_t bytes = 0;
+  for (const StreamSeg& sg : bs.segs)
+    if (!sg.ident) bytes += (size_t)sg.total * 2;
+  const size_t budget = std::min<size_t>((size_t)32 << 30, device_free_bytes() / 3);
+  if (bytes > budget) return RVSP_SUCCESS;
+  for (size_t j = 0; j < bs.segs.size(); ++j)
+    if (!bs.segs[j].ident && bs.segs[j].total > 0)
+      // 256 entries of slack: the map pass loads whole windows ahead.
+      RVSP_TRY(ensure(bs.parts[j].map, (size_t)bs.segs[j].total * 2 + 512));
+  bs.map_ready = true;  // the table now points at the maps
+  bs.segs_stale = true;
+  RVSP_TRY(upload_segments(bs, C, s));
+  const int grid = stream_grid(kStreamBuildPass, bs.build_capw, bs.stream_items);
+  RVSP_HIP_TRY(launch_stream(kStreamBuildPass, (const StreamSeg*)bs.d_segs.p, (int)bs.segs.size(),
+                             bs.stream_items, stacked_a(bs), stacked_b(bs),
+                             (StreamQueue*)bs.d_queue.p, bs.build_capw, grid, s));
+  for (size_t j = 0; j < bs.segs.size(); ++j)
+    if (!bs.segs[j].ident)
+      RVSP_HIP_TRY(launch_stream_rounds(bs.segs[j].map, bs.segs[j].total, s));
   unsigned bad = 0;
   RVSP_HIP_TRY(hipMemcpyAsync(&bad, (char*)bs.d_queue.p + offsetof(StreamQueue, bad),
                               sizeof(bad), hipMemcpyDeviceToHost, s));
   RVSP_HIP_TRY(hipStreamSynchronize(s));
-  if (bad) return RVSP_ERROR_DEVICE;  // a map that does not match the structure
-  bs.stream_items = item0;
-  bs.stream_capw = std::min(capw, kStreamNumericRow);
-  bs.stream_grid = stream_grid(false, bs.stream_capw, item0);
-  for (const Pick& pk : picks) bs.groups[pk.group].inner->stream_bins |= pk.bins;
-  bs.stream_failed = false;
-  bs.stream_ready = true;
+  if (bad) {  // a map that does not match the structure: keep searching
+    bs.map_ready = false;
+    bs.segs_stale = true;
+    return RVSP_ERROR_DEVICE;
+  }
+  bs.map_failed = false;
   return RVSP_SUCCESS;
 }
 
@@ -552,16 +606,16 @@ bool stream_verify() {
   return on;
 }
 
-rvsp_status_t launch_stream_numeric(BatchState& bs, hipStream_t s) {
-  DevCsr A, B;
-  A.ci = (const int*)bs.a_ci.p;
-  A.val = (const float*)bs.a_val.p;
-  B.rp = (const int*)bs.b_rp.p;
-  B.ci = (const int*)bs.b_ci.p;
-  B.val = (const float*)bs.b_val.p;
-  RVSP_HIP_TRY(launch_stream(false, (const StreamSeg*)bs.d_segs.p, (int)bs.segs.size(),
-                             bs.stream_items, A, B, (StreamQueue*)bs.d_queue.p, bs.stream_capw,
-                             bs.stream_grid, s));
+// One persistent launch over every segment: the map pass once the map is
+// built, the search pass before (first compute) or without it.
+rvsp_status_t launch_stream_numeric(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream_t s) {
+  RVSP_TRY(upload_segments(bs, C, s));
+  const bool use_map = bs.map_ready;
+  RVSP_HIP_TRY(launch_stream(use_map ? kStreamMapPass : kStreamSearchPass,
+                             (const StreamSeg*)bs.d_segs.p, (int)bs.segs.size(), bs.stream_items,
+                             stacked_a(bs), stacked_b(bs), (StreamQueue*)bs.d_queue.p,
+                             use_map ? bs.map_capw : bs.search_capw,
+                             use_map ? bs.map_grid : bs.search_grid, s));
   return RVSP_SUCCESS;
 }
 
@@ -632,8 +686,8 @@ rvsp_status_t batch_body(BatchState& bs, const rvsp_csr_matrix_t* C, bool full_o
       RVSP_HIP_TRY(hipStreamWaitEvent(bs.lane[k], bs.fork_ev, 0));
   }
   // Numeric-only computes: every group's streamed bins in one launch (the
-  // groups' own computes skip them).
-  if (bs.stream_ready && !full_out) RVSP_TRY(launch_stream_numeric(bs, s));
+  // groups' own computes skip them), overlapping the groups' other bins.
+  if (bs.stream_ready && !full_out) RVSP_TRY(launch_stream_numeric(bs, C, s));
   int gi = 0;
   for (BatchGroup& g : bs.groups) {
     const int lane = conc ? gi % BatchState::kLanes : 0;
@@ -655,6 +709,8 @@ rvsp_status_t batch_body(BatchState& bs, const rvsp_csr_matrix_t* C, bool full_o
       RVSP_HIP_TRY(hipStreamWaitEvent(s, bs.join_ev[k], 0));
     }
   }
+  // A full compute: the streamed bins once every group's columns are filled.
+  if (bs.stream_ready && full_out) RVSP_TRY(launch_stream_numeric(bs, C, s));
   RVSP_TRY(launch_deferred(bs, s));
   const SegTable& out = full_out ? bs.t_out_full : bs.t_out_vals;
   if (!out.chunks.empty()) RVSP_HIP_TRY(out.launch(s));
@@ -903,7 +959,7 @@ rvsp_status_t rvsp_spgemm_batch_compute(rvsp_spgemm_descr_t descr, int32_t count
   std::vector<const void*> ok = output_key(count, C);
   if (ok != bs.out_key) {
     bs.drop_graph();
-    bs.drop_stream();
+    bs.segs_stale = true;
     for (BatchGroup& g : bs.groups) {
       bool in_place = true;
       for (int p = g.p0 + 1; p < g.p1 && in_place; ++p) {
@@ -930,13 +986,20 @@ rvsp_status_t rvsp_spgemm_batch_compute(rvsp_spgemm_descr_t descr, int32_t count
   const bool repeat = bs.out_filled && key == bs.last_key;
   bs.last_key = key;
   bs.repeats = repeat ? bs.repeats + 1 : 0;
-  // The first numeric-only compute sets up the persistent row-stream pass
-  // (structure and output arrays are those of the last full compute).
-  if (repeat && d->batch_persistent && !bs.stream_ready && !bs.stream_failed) {
-    const rvsp_status_t st = build_stream(bs, C, s);
-    if (st != RVSP_SUCCESS) {
+  // The persistent row-stream pass: segments once per structure (before the
+  // first compute), the map at the first numeric-only compute.
+  if (d->batch_persistent && !bs.stream_ready && !bs.stream_failed) {
+    if (prepare_stream(bs, s) != RVSP_SUCCESS) {
       bs.drop_stream();
       bs.stream_failed = true;
+      (void)hipGetLastError();
+    }
+  }
+  if (repeat && bs.stream_ready && !bs.map_ready && !bs.map_failed) {
+    if (build_stream_map(bs, C, s) != RVSP_SUCCESS) {
+      bs.map_ready = false;
+      bs.map_failed = true;
+      bs.segs_stale = true;
       (void)hipGetLastError();
     }
   }

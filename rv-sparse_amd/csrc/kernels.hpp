@@ -279,7 +279,11 @@ struct StreamSeg {
   int nrows = 0;
   int row_base = 0;
   int c_nnz = 0;                    // entries of the C arrays the rows index
+  int ident = 0;                    // rows fed by one B row: positions need no map
 };
+constexpr int kStreamMapPass = 0;
+constexpr int kStreamBuildPass = 1;
+constexpr int kStreamSearchPass = 2;
 // Work queue of one launch; next and done are zero before the first launch,
 // and every launch leaves them zero again (its last wave resets them).
 // `bad` collects kStreamBad* bits of writes the kernels refused because
@@ -304,6 +308,8 @@ constexpr int kStreamMaxRow = 2048;
 // Accumulators per wave of the numeric pass; longer rows are walked once
 // per range of this many positions.
 constexpr int kStreamNumericRow = 1024;
+// The same for the search pass (accumulators, columns and order tags).
+constexpr int kStreamSearchRow = 512;
 constexpr int kStreamMaxSegs = 256;
 constexpr int kStreamPosBits = 11;
 // w[i] = ops[row of list position i].
@@ -320,13 +326,13 @@ hipError_t launch_stream_mark(const int* rows, int row_base, int nrows, int* fla
 // list[pos[i]] = i for rows i < M with flags[i] (pos: exclusive scan of flags).
 hipError_t launch_stream_compact(const int* flags, const int* pos, int M, int* list,
                                  hipStream_t s);
-// Waves of a persistent launch: as many as the device holds at once (one
-// wave per workgroup), at most `items`.
-int stream_grid(bool build, int capw, int items);
-// build: every map segment's entries from C's columns (positions and row
-// marks); otherwise the numeric pass. capw: the longest row of any map
-// segment; grid from stream_grid.
-hipError_t launch_stream(bool build, const StreamSeg* segs, int nseg, int items, const DevCsr& A,
+// Waves of a persistent launch of the given kStream*Pass: as many as the
+// device holds at once (one wave per workgroup), at most `items`.
+int stream_grid(int mode, int capw, int items);
+// One persistent launch of a kStream*Pass over the segments. capw: the
+// longest row a sub-batch's accumulators (or columns) hold; the build needs
+// room for every row of its segments; grid from stream_grid.
+hipError_t launch_stream(int mode, const StreamSeg* segs, int nseg, int items, const DevCsr& A,
                          const DevCsr& B, StreamQueue* q, int capw, int grid, hipStream_t s);
 // Order bits of a map segment's entries (after the build).
 hipError_t launch_stream_rounds(unsigned short* map, long long total, hipStream_t s);

@@ -154,14 +154,46 @@ __device__ __forceinline__ void stream_apply(float* vals, bool pv, int at, unsig
   }
 }
 
-template <bool BUILD>
+// MODE kStreamMapPass: positions from the map. kStreamBuildPass: writes the
+// map (positions by binary search in the rows' C columns). kStreamSearchPass:
+// positions by the same search, adds ordered at run time (no map needed).
+// The search pass's adds: products of one window that hit the same
+// accumulator apply in lane order (= product order). Each round, every
+// pending lane proposes its lane id to its position's tag (an LDS atomic
+// minimum); the lowest proposer of each position adds and empties the tag.
+// One round when no position repeats in the window.
+__device__ __forceinline__ void search_apply(float* vals, int* tags, bool use, int at, float a,
+                                             float b) {
+  const int lane = threadIdx.x & (kWave - 1);
+  bool act = use;
+  while (__ballot(act)) {
+    if (act) atomicMin(&tags[at], lane);
+    wave_lds_order();
+    const bool won = act && tags[at] == lane;
+    if (won) {
+      vals[at] = acc_step(vals[at], a, b);
+      tags[at] = kWave;
+    }
+    wave_lds_order();
+    act = act && !won;
+  }
+}
+
+template <int MODE>
 __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ segs, int nseg,
                                                   int items, DevCsr A, DevCsr B,
                                                   StreamQueue* __restrict__ q, int capw) {
+  constexpr bool BUILD = MODE == kStreamBuildPass;
+  constexpr bool SEARCH = MODE == kStreamSearchPass;
   extern __shared__ __attribute__((aligned(16))) int st_lds[];
-  float* vals = reinterpret_cast<float*>(st_lds);  // numeric: accumulators of the sub-batch
-  int* cols = st_lds;                               // build: C columns of the sub-batch
-  int* mark = st_lds + ((capw + 3) & ~3);
+  const int cap4 = (capw + 3) & ~3;
+  // Numeric passes: accumulators of the sub-batch; the build: its rows' C
+  // columns; the search pass: accumulators, columns and one order tag per
+  // position.
+  float* vals = reinterpret_cast<float*>(st_lds);
+  int* cols = BUILD ? st_lds : st_lds + cap4;
+  int* tags = st_lds + 2 * cap4;
+  int* mark = st_lds + (BUILD ? 1 : SEARCH ? 3 : 1) * cap4;
   // Per A entry of the current chunk: B position of its product at chunk
   // offset 0, scale, and accumulator (or C) offset of its row.
   int4* etab = reinterpret_cast<int4*>(mark + kWave);
@@ -197,7 +229,7 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
     for (int j = 0; j < kSegLanes; ++j) sidx += __popcll(__ballot(seg_i0[j] <= item));
     sidx = __builtin_amdgcn_readfirstlane(sidx);
     const StreamSeg& S = segs[sidx];
-    const bool ident = S.map == nullptr;
+    const bool ident = S.ident != 0;
     if (BUILD && ident) continue;
     const gp<const int> rows = gload(S.rows);
     const bool listed = S.rows != nullptr;
@@ -256,20 +288,26 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
       for (int pass = 0; pass < npass; ++pass) {
         const int p0 = pass * capw;  // first position of this walk
         const int pn = npass > 1 ? min(capw, totn - p0) : totn;  // accumulators in use
+        int clo = 0, chi = 0;  // search pass over one long row: columns of this walk
         if (!ident) {
-          // Accumulators cleared (numeric) or the rows' columns staged (build).
+          // Accumulators cleared (numeric passes), the rows' columns of this
+          // walk staged (build, search) and the order tags emptied (search).
           int rc = 0;
           for (int j0 = 0; j0 < pn; j0 += kWave) {
-            if (!BUILD) {
-              if (j0 + lane < pn) vals[j0 + lane] = 0.0f;
-              continue;
-            }
+            const int j = j0 + lane;
+            if (!BUILD && j < pn) vals[j] = 0.0f;
+            if (SEARCH && j < pn) tags[j] = kWave;
+            if (!BUILD && !SEARCH) continue;
             const int r = wave_owner(mark, rl && n > 0, abase, j0, rc);
             rc = wave_last(r);
-            const int src = __shfl(cs, r) + j0 + lane - __shfl(abase, r);
-            if (j0 + lane < totn) cols[j0 + lane] = gload(S.c_ci)[src];
+            const int src = __shfl(cs, r) + p0 + j - __shfl(abase, r);
+            if (j < pn) cols[j] = gload(S.c_ci)[src];
           }
           wave_lds_order();
+          if (SEARCH && npass > 1) {
+            clo = cols[0];
+            chi = cols[pn - 1];
+          }
         }
         long long qc = __shfl(rq, 0);  // stream index of the chunk's first product
         int ec = 0;
@@ -347,9 +385,13 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
               // base + t, base = the row's C start minus the chunk offset of
               // the row's first product.
               const int e_base = ident ? e_cs - (int)(e_rq - qc) : e_acc;
-              etab[lane] = make_int4(boff, __float_as_int(a), e_base, 0);
+              // Columns the search pass looks a product up in: its row's
+              // (or, walking one long row, this walk's).
+              const int e_len = npass > 1 ? pn : e_n;
+              etab[lane] = make_int4(boff, __float_as_int(a), e_base, e_len);
               wave_lds_order();
-              auto step = [&](int k, bool& pv, int& t, int& bi, float& av, int& base) {
+              auto step = [&](int k, bool& pv, int& t, int& bi, float& av, int& base,
+                              int& nlen) {
                 const int off = (k << 6) - lead;
                 const int e = wave_owner(mark, len > 0, excl_p, off, pc);
                 pc = wave_last(e);
@@ -359,13 +401,14 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
                 bi = et.x + t;
                 av = __int_as_float(et.y);
                 base = et.z;
+                nlen = et.w;
               };
               if (ident) {
                 for (int k = 0; k < nwin; ++k) {
                   bool pv;
-                  int t, bi, base;
+                  int t, bi, base, nlen;
                   float av;
-                  step(k, pv, t, bi, av, base);
+                  step(k, pv, t, bi, av, base, nlen);
                   const int dst = base + t;
                   if (pv && (dst < 0 || dst >= c_nnz)) bad |= kStreamBadValue;
                   else if (pv) c_val[dst] = acc_step(0.0f, av, B.val[bi]);
@@ -375,26 +418,37 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
                 // applied. Their loads are unconditional, also past the last
                 // window (no products there; map entries in the slack), so no
                 // branch makes the compiler wait for them early.
+                // x: the product's map entry (map pass) or B column (search).
                 struct Win {
                   bool pv;
-                  int acc;
+                  int acc, nlen;
                   float av, bv;
-                  unsigned me;
+                  unsigned x;
                 };
                 auto issue = [&](int k) {
                   Win w;
                   int t, bi;
-                  step(k, w.pv, t, bi, w.av, w.acc);
-                  w.me = mc[(k << 6) + lane];
-                  w.bv = B.val[w.pv ? bi : 0];
+                  step(k, w.pv, t, bi, w.av, w.acc, w.nlen);
+                  const int bq = w.pv ? bi : 0;
+                  w.x = SEARCH ? (unsigned)B.ci[bq] : (unsigned)mc[(k << 6) + lane];
+                  w.bv = B.val[bq];
                   return w;
                 };
                 auto apply = [&](const Win& w) {
-                  const int rel = (int)(w.me & kPosMask) - p0;
-                  const int at = w.acc + rel;
-                  const bool use = w.pv && rel >= 0 && at < pn;
-                  if (npass == 1 && w.pv && !use) bad |= kStreamBadLds;
-                  stream_apply(vals, use, at, w.me >> kOrderShift, w.av, w.bv);
+                  if (SEARCH) {
+                    const int col = (int)w.x;
+                    const bool use = w.pv && (npass == 1 || (col >= clo && col <= chi));
+                    const int pos = lds_find(cols + w.acc, use ? w.nlen : 1, col);
+                    const int at = w.acc + pos;
+                    if (use && (at >= pn || cols[at] != col)) bad |= kStreamBadColumn;
+                    search_apply(vals, tags, use && at < pn, at, w.av, w.bv);
+                  } else {
+                    const int rel = (int)(w.x & kPosMask) - p0;
+                    const int at = w.acc + rel;
+                    const bool use = w.pv && rel >= 0 && at < pn;
+                    if (npass == 1 && w.pv && !use) bad |= kStreamBadLds;
+                    stream_apply(vals, use, at, w.x >> kOrderShift, w.av, w.bv);
+                  }
                 };
                 // Unrolled by three so the windows' registers change roles
                 // without copies (a copy of a load's destination would wait
@@ -473,16 +527,23 @@ __global__ __launch_bounds__(kBlock) void k_stream_rounds(unsigned short* __rest
   if (v) map[qq] = (unsigned short)((ent & (kPosMask | kRowMark)) | (ord << kOrderShift));
 }
 
-// Accumulators (or columns) of capw positions, rounded to 16 bytes, the
-// marks and the entry table.
-int stream_lds_bytes(int capw) { return ((capw + 3) & ~3) * 4 + kWave * 4 + kWave * 16; }
+// LDS of one wave: capw positions (rounded to 16 bytes) of accumulators
+// (map pass), columns (build) or accumulators, columns and tags (search),
+// then the marks and the entry table.
+int stream_lds_bytes(int mode, int capw) {
+  const int arrays = mode == kStreamSearchPass ? 3 : 1;
+  return arrays * ((capw + 3) & ~3) * 4 + kWave * 4 + kWave * 16;
+}
 
-int stream_waves_per_cu(bool build, int lds) {
+int stream_waves_per_cu(int mode, int lds) {
   int nb = 0;
-  const hipError_t e = build ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_stream<true>,
-                                                                            kWave, lds)
-                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_stream<false>,
-                                                                            kWave, lds);
+  hipError_t e;
+  if (mode == kStreamBuildPass)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_stream<kStreamBuildPass>, kWave, lds);
+  else if (mode == kStreamSearchPass)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_stream<kStreamSearchPass>, kWave, lds);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_stream<kStreamMapPass>, kWave, lds);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return 8;
@@ -508,7 +569,7 @@ hipError_t launch_stream_items(const long long* soff, int nrows, int items, int*
   return hipGetLastError();
 }
 
-int stream_grid(bool build, int capw, int items) {
+int stream_grid(int mode, int capw, int items) {
   if (items <= 0 || capw < 0 || capw > kStreamMaxRow) return 0;
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
@@ -516,7 +577,7 @@ int stream_grid(bool build, int capw, int items) {
     (void)hipGetLastError();
     cus = 256;
   }
-  const long long fit = (long long)stream_waves_per_cu(build, stream_lds_bytes(capw)) * cus;
+  const long long fit = (long long)stream_waves_per_cu(mode, stream_lds_bytes(mode, capw)) * cus;
   return (int)(fit < items ? fit : items);
 }
 
@@ -535,18 +596,21 @@ hipError_t launch_stream_compact(const int* flags, const int* pos, int M, int* l
   return hipGetLastError();
 }
 
-hipError_t launch_stream(bool build, const StreamSeg* segs, int nseg, int items, const DevCsr& A,
+hipError_t launch_stream(int mode, const StreamSeg* segs, int nseg, int items, const DevCsr& A,
                          const DevCsr& B, StreamQueue* q, int capw, int grid, hipStream_t s) {
   if (items <= 0 || nseg <= 0) return hipSuccess;
   if (nseg > kStreamMaxSegs || capw < 0 || capw > kStreamMaxRow || grid <= 0)
     return hipErrorInvalidValue;
-  const int lds = stream_lds_bytes(capw);
-  if (build)
-    hipLaunchKernelGGL(k_stream<true>, dim3(grid), dim3(kWave), lds, s, segs, nseg, items, A, B,
-                       q, capw);
+  const int lds = stream_lds_bytes(mode, capw);
+  if (mode == kStreamBuildPass)
+    hipLaunchKernelGGL(k_stream<kStreamBuildPass>, dim3(grid), dim3(kWave), lds, s, segs, nseg,
+                       items, A, B, q, capw);
+  else if (mode == kStreamSearchPass)
+    hipLaunchKernelGGL(k_stream<kStreamSearchPass>, dim3(grid), dim3(kWave), lds, s, segs, nseg,
+                       items, A, B, q, capw);
   else
-    hipLaunchKernelGGL(k_stream<false>, dim3(grid), dim3(kWave), lds, s, segs, nseg, items, A,
-                       B, q, capw);
+    hipLaunchKernelGGL(k_stream<kStreamMapPass>, dim3(grid), dim3(kWave), lds, s, segs, nseg,
+                       items, A, B, q, capw);
   return hipGetLastError();
 }
 
