@@ -194,18 +194,26 @@ rvsp_status_t rvsp_spgemm_batch_set_graph(rvsp_spgemm_descr_t descr, int on);
  * bits are the same either way. Fusing saves launches on many small groups
  * but gives up the overlap. Must precede the batch estimation. */
 rvsp_status_t rvsp_spgemm_batch_set_fuse(rvsp_spgemm_descr_t descr, int on);
-/* on != 0 (default): from the first numeric-only batch compute on, every
- * group's numeric bin of rows up to 2048 entries long (and of rows fed by a
- * single B row) runs as one persistent launch over a product-balanced work
- * queue of (group, bin, row block) items, captured with the rest of the
- * compute in the batch's graph; the position map it reads (16 bits per
- * product) is built by the first numeric-only compute. 0: one launch per bin
- * and group. Output bits are the same either way. No reference counterpart
- * (batching is not part of the reference). */
-rvsp_status_t rvsp_spgemm_batch_set_persistent(rvsp_spgemm_descr_t descr, int on);
+/* How batch computes run the groups' numeric bins of rows up to 2048 entries
+ * long (and of rows fed by a single B row): as one persistent launch over a
+ * product-balanced work queue of (group, bin, row block) items, captured with
+ * the rest of a numeric-only compute in the batch's graph; the first compute
+ * finds each product's position by a search of its C row, numeric-only
+ * computes read a position map (16 bits per product) built by the first of
+ * them. mode 1 (default): the first numeric-only compute also times the
+ * groups' per-bin launches and keeps the faster path; 2: always the
+ * persistent launch; 0: per-bin launches. Output bits are the same in every
+ * mode. No reference counterpart (batching is not part of the reference). */
+rvsp_status_t rvsp_spgemm_batch_set_persistent(rvsp_spgemm_descr_t descr, int mode);
 /* Segments (group, bin) served by the persistent launch; 0 before the
  * first numeric-only compute or when it is off. */
 int32_t rvsp_spgemm_batch_stream_segments(rvsp_spgemm_descr_t descr);
+/* The numeric path of the batch's computes: 1 the persistent launch, 2 the
+ * per-bin launches after measuring them faster, 0 neither (mode 0, or not
+ * estimated). The times the first numeric-only compute measured (0 when it
+ * made no choice) go to *persistent_ms and *per_bin_ms when not NULL. */
+int32_t rvsp_spgemm_batch_numeric_path(rvsp_spgemm_descr_t descr, float* persistent_ms,
+                                       float* per_bin_ms);
 
 /* Caps the product count of a group (default and maximum INT32_MAX; 0
  * restores the default). Must precede the batch estimation. */

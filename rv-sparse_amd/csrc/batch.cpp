@@ -129,11 +129,16 @@ struct BatchState {
   bool stream_ready = false, stream_failed = false;  // segments prepared
   bool map_ready = false, map_failed = false;        // map built (or refused)
   bool segs_stale = true;  // output pointers changed since the last upload
+  // The first numeric-only compute times the persistent pass against the
+  // groups' own per-bin launches (each with its position map) and keeps the
+  // faster; declined stays until the structure changes.
+  bool path_chosen = false, stream_declined = false;
+  float chosen_ms[2] = {0.0f, 0.0f};  // measured: persistent, per bin
   int graph_launches = 0;
   int repeats = 0;  // consecutive numeric-only computes with the same arrays
-  // Eager computes run the groups on kLanes streams at once (fork and join
-  // through events on the descriptor's stream), so one group's long-row
-  // tail overlaps the next group's work; captured computes use one stream.
+  // Computes run the groups on kLanes streams at once (fork and join
+  // through events on the descriptor's stream, also inside a captured
+  // graph), so one group's long-row tail overlaps the next group's work.
   // Two lanes: more streams than the process's hardware queues (four by
   // default, shared with the caller's streams) made some computes stall.
   static constexpr int kLanes = kMaxLanes;
@@ -202,6 +207,8 @@ struct BatchState {
     map_ready = false;
     map_failed = false;
     segs_stale = true;
+    path_chosen = false;
+    stream_declined = false;
     stream_items = 0;
   }
   ~BatchState() {
@@ -488,15 +495,7 @@ rvsp_status_t prepare_stream(BatchState& bs, hipStream_t s) {
                               hipMemcpyDeviceToHost, s));
   RVSP_HIP_TRY(hipStreamSynchronize(s));
   long long items = 0;
-  int capw = 0;
-  for (int j = 0; j < ns; ++j) {
-    if (!picks[j].ident) {
-      const PhasePlan& p = bs.groups[picks[j].group].inner->numeric;
-      for (int b = 2; b < p.table.rule.nbins; ++b)
-        if ((picks[j].bins >> b) & 1u) capw = std::max(capw, p.table.rule.bound[b]);
-    }
-    items += (totals[j] + kStreamItem - 1) / kStreamItem;
-  }
+  for (int j = 0; j < ns; ++j) items += (totals[j] + kStreamItem - 1) / kStreamItem;
   if (items > INT_MAX / 2) return RVSP_SUCCESS;
   bs.segs.assign(ns, StreamSeg());
   bs.seg_group.assign(ns, 0);
@@ -529,9 +528,11 @@ rvsp_status_t prepare_stream(BatchState& bs, hipStream_t s) {
   RVSP_HIP_TRY(hipMemsetAsync(bs.d_queue.p, 0, sizeof(StreamQueue), s));
   RVSP_HIP_TRY(hipStreamSynchronize(s));
   bs.stream_items = item0;
-  bs.build_capw = std::min(capw, kStreamMaxRow);
-  bs.map_capw = std::min(bs.build_capw, kStreamNumericRow);
-  bs.search_capw = std::min(bs.build_capw, kStreamSearchRow);
+  // Accumulators of the full sizes whatever the rows' bound, so a sub-batch
+  // holds as many short rows as fit; the build holds every row whole.
+  bs.build_capw = kStreamMaxRow;
+  bs.map_capw = kStreamNumericRow;
+  bs.search_capw = kStreamSearchRow;
   bs.map_grid = stream_grid(kStreamMapPass, bs.map_capw, item0);
   bs.search_grid = stream_grid(kStreamSearchPass, bs.search_capw, item0);
   for (const Pick& pk : picks) bs.groups[pk.group].inner->stream_bins |= pk.bins;
@@ -619,6 +620,66 @@ rvsp_status_t launch_stream_numeric(BatchState& bs, const rvsp_csr_matrix_t* C, 
   return RVSP_SUCCESS;
 }
 
+rvsp_status_t batch_body(BatchState& bs, const rvsp_csr_matrix_t* C, bool full_out,
+                         hipStream_t s, bool concurrent);
+
+// The first numeric-only compute with the persistent pass ready: one compute
+// through it, then the groups' per-bin launches (one compute that builds
+// their position maps, one timed), and the faster path stays; the other
+// path's maps are released. Every compute writes the same values.
+rvsp_status_t choose_numeric_path(BatchState& bs, const rvsp_csr_matrix_t* C, hipStream_t s) {
+  hipEvent_t ev[4] = {};
+  for (hipEvent_t& e : ev) RVSP_HIP_TRY(hipEventCreate(&e));
+  auto destroy = [&] {
+    for (hipEvent_t& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  };
+  std::vector<unsigned> masks;
+  rvsp_status_t st = RVSP_SUCCESS;
+  hipError_t he = hipEventRecord(ev[0], s);
+  if (he == hipSuccess) st = batch_body(bs, C, false, s, true);
+  if (he == hipSuccess && st == RVSP_SUCCESS) he = hipEventRecord(ev[1], s);
+  if (he == hipSuccess && st == RVSP_SUCCESS) {
+    for (BatchGroup& g : bs.groups) {
+      masks.push_back(g.inner->stream_bins);
+      g.inner->stream_bins = 0;
+      g.inner->map_promoted = false;  // the next numeric-only compute promotes
+    }
+    bs.stream_ready = false;
+    st = batch_body(bs, C, false, s, true);
+    if (st == RVSP_SUCCESS) he = hipEventRecord(ev[2], s);
+    if (st == RVSP_SUCCESS && he == hipSuccess) st = batch_body(bs, C, false, s, true);
+    if (st == RVSP_SUCCESS && he == hipSuccess) he = hipEventRecord(ev[3], s);
+    if (he == hipSuccess) he = hipEventSynchronize(ev[3]);
+  }
+  float t_stream = 0.0f, t_bin = 0.0f;
+  if (he == hipSuccess && st == RVSP_SUCCESS) he = hipEventElapsedTime(&t_stream, ev[0], ev[1]);
+  if (he == hipSuccess && st == RVSP_SUCCESS) he = hipEventElapsedTime(&t_bin, ev[2], ev[3]);
+  destroy();
+  if (masks.size() == bs.groups.size()) {
+    // Back to the persistent pass unless the per-bin path was measured faster.
+    const bool keep = he != hipSuccess || st != RVSP_SUCCESS || t_stream <= t_bin;
+    if (keep) {
+      for (size_t i = 0; i < bs.groups.size(); ++i) {
+        bs.groups[i].inner->stream_bins = masks[i];
+        RVSP_TRY(release_position_map(bs.groups[i].inner));
+      }
+      bs.stream_ready = true;
+      bs.segs_stale = true;
+    } else {
+      bs.drop_stream();  // releases the persistent pass's maps
+      bs.stream_declined = true;
+      // The groups' promoted bins stay on their maps.
+      for (BatchGroup& g : bs.groups) g.inner->map_promoted = true;
+    }
+  }
+  bs.path_chosen = true;
+  bs.chosen_ms[0] = t_stream;
+  bs.chosen_ms[1] = t_bin;
+  RVSP_HIP_TRY(he);
+  return st;
+}
+
 // The groups' long-row numeric bins (workgroup tiles), deferred by their
 // computes: one fused launch per tile size and up to kMaxFusedSegs bins, on
 // the descriptor's stream after every group's fill and other bins.
@@ -685,9 +746,6 @@ rvsp_status_t batch_body(BatchState& bs, const rvsp_csr_matrix_t* C, bool full_o
     for (int k = 0; k < BatchState::kLanes - 1; ++k)
       RVSP_HIP_TRY(hipStreamWaitEvent(bs.lane[k], bs.fork_ev, 0));
   }
-  // Numeric-only computes: every group's streamed bins in one launch (the
-  // groups' own computes skip them), overlapping the groups' other bins.
-  if (bs.stream_ready && !full_out) RVSP_TRY(launch_stream_numeric(bs, C, s));
   int gi = 0;
   for (BatchGroup& g : bs.groups) {
     const int lane = conc ? gi % BatchState::kLanes : 0;
@@ -703,6 +761,11 @@ rvsp_status_t batch_body(BatchState& bs, const rvsp_csr_matrix_t* C, bool full_o
     char token = 0;
     RVSP_TRY(rvsp_spgemm_compute(g.inner, &Ast, &Bst, &Cg, &token));
   }
+  // Numeric-only computes: every group's streamed bins in one launch (the
+  // groups' own computes skip them), queued behind the groups' other
+  // (long-row) launches so those get their CUs before the persistent waves
+  // take the rest.
+  if (bs.stream_ready && !full_out) RVSP_TRY(launch_stream_numeric(bs, C, s));
   if (conc) {
     for (int k = 0; k < BatchState::kLanes - 1; ++k) {
       RVSP_HIP_TRY(hipEventRecord(bs.join_ev[k], bs.lane[k]));
@@ -988,7 +1051,7 @@ rvsp_status_t rvsp_spgemm_batch_compute(rvsp_spgemm_descr_t descr, int32_t count
   bs.repeats = repeat ? bs.repeats + 1 : 0;
   // The persistent row-stream pass: segments once per structure (before the
   // first compute), the map at the first numeric-only compute.
-  if (d->batch_persistent && !bs.stream_ready && !bs.stream_failed) {
+  if (d->batch_persistent && !bs.stream_ready && !bs.stream_failed && !bs.stream_declined) {
     if (prepare_stream(bs, s) != RVSP_SUCCESS) {
       bs.drop_stream();
       bs.stream_failed = true;
@@ -1009,7 +1072,9 @@ rvsp_status_t rvsp_spgemm_batch_compute(rvsp_spgemm_descr_t descr, int32_t count
     if (!bs.exec) {
       // Capture one numeric-only compute; later ones replay it.
       RVSP_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      const rvsp_status_t st = batch_body(bs, C, false, s, false);
+      // The lanes are forked from and joined into the captured stream by
+      // events, so the graph keeps their overlap.
+      const rvsp_status_t st = batch_body(bs, C, false, s, true);
       hipGraph_t g = nullptr;
       const hipError_t ec = hipStreamEndCapture(s, &g);
       hipGraphExec_t ex = nullptr;
@@ -1031,6 +1096,9 @@ rvsp_status_t rvsp_spgemm_batch_compute(rvsp_spgemm_descr_t descr, int32_t count
     } else {
       RVSP_TRY(batch_body(bs, C, false, s, true));
     }
+  } else if (repeat && bs.stream_ready && bs.map_ready && !bs.path_chosen &&
+             d->batch_persistent == 1) {
+    RVSP_TRY(choose_numeric_path(bs, C, s));
   } else {
     RVSP_TRY(batch_body(bs, C, !bs.out_filled, s, true));
     bs.out_filled = true;
@@ -1069,12 +1137,15 @@ rvsp_status_t rvsp_spgemm_batch_set_graph(rvsp_spgemm_descr_t descr, int on) {
   return RVSP_SUCCESS;
 }
 
-rvsp_status_t rvsp_spgemm_batch_set_persistent(rvsp_spgemm_descr_t descr, int on) {
+rvsp_status_t rvsp_spgemm_batch_set_persistent(rvsp_spgemm_descr_t descr, int mode) {
   if (descr == NULL) return RVSP_ERROR_NULL_POINTER;
-  descr->batch_persistent = on != 0;
+  if (mode < 0 || mode > 2) return RVSP_ERROR_INVALID_ARGUMENT;
+  descr->batch_persistent = mode;
   if (descr->batch) {
     descr->batch->drop_graph();
     descr->batch->drop_stream();
+    for (BatchGroup& g : descr->batch->groups)
+      if (g.inner) g.inner->map_promoted = false;  // per-bin maps are promoted anew
   }
   return RVSP_SUCCESS;
 }
@@ -1082,6 +1153,16 @@ rvsp_status_t rvsp_spgemm_batch_set_persistent(rvsp_spgemm_descr_t descr, int on
 int32_t rvsp_spgemm_batch_stream_segments(rvsp_spgemm_descr_t descr) {
   if (descr == NULL || !descr->batch || !descr->batch->stream_ready) return 0;
   return (int32_t)descr->batch->segs.size();
+}
+
+int32_t rvsp_spgemm_batch_numeric_path(rvsp_spgemm_descr_t descr, float* persistent_ms,
+                                       float* per_bin_ms) {
+  if (descr == NULL || !descr->batch) return 0;
+  const BatchState& bs = *descr->batch;
+  if (persistent_ms) *persistent_ms = bs.chosen_ms[0];
+  if (per_bin_ms) *per_bin_ms = bs.chosen_ms[1];
+  if (bs.stream_declined) return 2;
+  return bs.stream_ready ? 1 : 0;
 }
 
 rvsp_status_t rvsp_spgemm_batch_set_fuse(rvsp_spgemm_descr_t descr, int on) {

@@ -740,6 +740,24 @@ rvsp_status_t ensure_map_built(rvsp_spgemm_descr* d, const DevCsr& A, const DevC
   return RVSP_SUCCESS;
 }
 
+rvsp_status_t release_position_map(rvsp_spgemm_descr* d) {
+  PhaseTable& t = d->numeric.table;
+  for (int b = 2; b < t.rule.nbins; ++b)
+    if (t.spec[b].kind == KIND_MAP) {
+      BinChoice c;
+      c.kind = KIND_SEARCH;
+      apply_choice(t, PHASE_NUMERIC, b, c, t.rule.bound[b]);
+    }
+  release(d->map);
+  release(d->poff);
+  release(d->map_weights);
+  d->map_valid = false;
+  d->map_bins = 0;
+  d->map_width = 0;
+  d->map_promoted = false;
+  return RVSP_SUCCESS;
+}
+
 constexpr int kMapFromTiledMax = 2048;  // longest C row a tiled bin may move to the map with
 
 rvsp_status_t promote_to_map(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,

@@ -147,9 +147,10 @@ struct rvsp_spgemm_descr {
   // persistent row-stream launch (run_numeric skips them, and they never move
   // to the position map).
   unsigned stream_bins = 0;
-  // Batch setting: numeric-only computes run every group's short-row numeric
-  // bins as one persistent row-stream launch.
-  bool batch_persistent = true;
+  // Batch setting (rvsp_spgemm_batch_set_persistent): 0 per-bin launches,
+  // 1 the persistent row-stream pass unless the first numeric-only compute
+  // measures the per-bin launches faster, 2 the persistent pass always.
+  int batch_persistent = 1;
 
   // Set by the one-shot path: work_estimation checks the staged operands
   // for canonical CSR on the device before any other kernel reads them.
@@ -184,6 +185,9 @@ rvsp_status_t plan_map(rvsp_spgemm_descr* d, hipStream_t s);
 rvsp_status_t ensure_split(rvsp_spgemm_descr* d, const DevCsr& B, hipStream_t s);
 rvsp_status_t ensure_map_built(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B,
                                const int* c_ci, hipStream_t s);
+// Drops the position map: its bins search C's columns again, and the next
+// numeric-only compute may promote them anew.
+rvsp_status_t release_position_map(rvsp_spgemm_descr* d);
 // Moves the non-empty searching numeric bins to the position map (chunked
 // variant, same lanes and capacity) and builds the map; bins over the memory
 // budget keep searching. Synchronous.

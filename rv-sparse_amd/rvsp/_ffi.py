@@ -138,6 +138,7 @@ PROTOTYPES = {
     "rvsp_spgemm_batch_set_fuse": (C.c_int, [P, C.c_int]),
     "rvsp_spgemm_batch_set_persistent": (C.c_int, [P, C.c_int]),
     "rvsp_spgemm_batch_stream_segments": (I32, [P]),
+    "rvsp_spgemm_batch_numeric_path": (I32, [P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "rvsp_hip_malloc": (C.c_int, [C.POINTER(P), SZ]),
     "rvsp_hip_free": (C.c_int, [P]),
     "rvsp_hip_memcpy": (C.c_int, [P, P, SZ, C.c_int]),

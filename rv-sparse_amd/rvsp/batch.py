@@ -13,7 +13,7 @@ class Batch:
     """A descriptor used for batched products C[p] = A[p] * B[p]."""
 
     def __init__(self, algo=F.ALGO_DEFAULT, group_limit=None, graph=True, choices=None,
-                 fuse=False, persistent=True):
+                 fuse=False, persistent=1):
         self.handle = C.c_void_p()
         check(lib().rvsp_spgemm_descr_create(C.byref(self.handle)), "descr_create")
         check(lib().rvsp_spgemm_set_algo(self.handle, algo), "set_algo")
@@ -27,12 +27,19 @@ class Batch:
                   "batch_set_group_limit")
         check(lib().rvsp_spgemm_batch_set_graph(self.handle, 1 if graph else 0), "set_graph")
         check(lib().rvsp_spgemm_batch_set_fuse(self.handle, 1 if fuse else 0), "set_fuse")
-        check(lib().rvsp_spgemm_batch_set_persistent(self.handle, 1 if persistent else 0),
-              "set_persistent")
+        # persistent: 0 per-bin launches, 1 measured choice, 2 always the
+        # persistent launch (True/False accepted for 2/0).
+        mode = {True: 2, False: 0}.get(persistent, persistent) if isinstance(persistent, bool) \
+            else int(persistent)
+        check(lib().rvsp_spgemm_batch_set_persistent(self.handle, mode), "set_persistent")
         self.c_nnz = None
 
     @staticmethod
     def _structs(mats):
+        """A ctypes CsrMatrix array for the operands (returned as is when it
+        already is one, so callers can build it once for repeated computes)."""
+        if isinstance(mats, C.Array):
+            return mats
         arr = (F.CsrMatrix * len(mats))()
         for i, m in enumerate(mats):
             arr[i] = m if isinstance(m, F.CsrMatrix) else m.struct()
@@ -59,6 +66,12 @@ class Batch:
 
     def stream_segments(self):
         return int(lib().rvsp_spgemm_batch_stream_segments(self.handle))
+
+    def numeric_path(self):
+        """(path, persistent_ms, per_bin_ms): path 1 persistent, 2 per bin."""
+        a, b = C.c_float(), C.c_float()
+        path = int(lib().rvsp_spgemm_batch_numeric_path(self.handle, C.byref(a), C.byref(b)))
+        return path, a.value, b.value
 
     def stream(self):
         s = C.c_void_p()
@@ -135,7 +148,7 @@ def upload_pairs(pairs):
 
 
 def spgemm_batch(pairs, algo=F.ALGO_DEFAULT, contiguous=True, group_limit=None, graph=True,
-                 repeats=1, choices=None, fuse=False, persistent=True):
+                 repeats=1, choices=None, fuse=False, persistent=1):
     """Host pairs in, host products out, through the batched device path."""
     As, Bs, owned = upload_pairs(pairs)
     bt = Batch(algo=algo, group_limit=group_limit, graph=graph, choices=choices, fuse=fuse,

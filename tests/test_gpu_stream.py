@@ -58,7 +58,7 @@ def assert_pairs_match(pairs, got):
 def test_stream_pass_matches_oracle_over_repeats(gpu, graph, algo):
     pairs = collision_pairs()
     As, Bs, owned = upload_pairs(pairs)
-    bt = Batch(algo=algo, graph=graph, group_limit=400_000)
+    bt = Batch(algo=algo, graph=graph, group_limit=400_000, persistent=True)
     out = None
     rng = np.random.default_rng(21)
     try:
@@ -115,3 +115,36 @@ def test_stream_pass_on_and_off_agree_on_uneven_pairs(gpu):
         assert g.values.tobytes() == h.values.tobytes()
         assert g.col_idx.tobytes() == h.col_idx.tobytes()
     assert_pairs_match(pairs, res[True])
+
+
+def test_measured_choice_between_persistent_and_per_bin_launches(gpu):
+    """Default mode: the first numeric-only compute times the persistent pass
+    against the groups' per-bin launches and keeps the faster; the bits are
+    the oracle's before, during and after the choice."""
+    pairs = gen.batch_pairs(30, n_min=64, n_max=4096, seed=41)
+    As, Bs, owned = upload_pairs(pairs)
+    bt = Batch(algo=F.ALGO_HIP_LEARNED, group_limit=600_000)
+    out = None
+    try:
+        nnz = bt.estimate(As, Bs)
+        assert bt.groups() >= 2
+        out = BatchOutput([a.rows for a, _ in pairs], [b.cols for _, b in pairs], nnz)
+        F.check(bt.compute(As, Bs, out.structs), "batch_compute")
+        assert bt.numeric_path()[0] == 1  # the first compute's search pass
+        assert_pairs_match(pairs, [out.download(p) for p in range(len(pairs))])
+        for _ in range(3):
+            for p in range(len(pairs)):
+                to_device(out.val[p], np.full(out.c_nnz[p], -1, np.int32))
+            F.check(bt.compute(As, Bs, out.structs), "batch_compute")
+            assert_pairs_match(pairs, [out.download(p) for p in range(len(pairs))])
+        path, t_persistent, t_bin = bt.numeric_path()
+        assert path in (1, 2)
+        assert t_persistent > 0 and t_bin > 0
+        assert (path == 1) == (t_persistent <= t_bin)
+        assert (bt.stream_segments() > 0) == (path == 1)
+    finally:
+        if out is not None:
+            out.free()
+        bt.close()
+        for d in owned:
+            d.free()
