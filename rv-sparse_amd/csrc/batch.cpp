@@ -665,7 +665,6 @@ rvsp_status_t choose_numeric_path(BatchState& bs, const rvsp_csr_matrix_t* C, hi
         RVSP_TRY(release_position_map(bs.groups[i].inner));
       }
       bs.stream_ready = true;
-      bs.segs_stale = true;
     } else {
       bs.drop_stream();  // releases the persistent pass's maps
       bs.stream_declined = true;
@@ -1070,6 +1069,12 @@ rvsp_status_t rvsp_spgemm_batch_compute(rvsp_spgemm_descr_t descr, int32_t count
   // their position maps, which synchronises); the second is captured.
   if (repeat && bs.repeats >= 2 && d->batch_graph && !bs.graph_failed) {
     if (!bs.exec) {
+      // The segment table goes to the device before the capture (a host
+      // copy inside it would be replayed from pageable memory each time).
+      if (bs.stream_ready && bs.segs_stale) {
+        RVSP_TRY(upload_segments(bs, C, s));
+        RVSP_HIP_TRY(hipStreamSynchronize(s));
+      }
       // Capture one numeric-only compute; later ones replay it.
       RVSP_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       // The lanes are forked from and joined into the captured stream by

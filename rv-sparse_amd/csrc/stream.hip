@@ -118,12 +118,15 @@ __device__ __forceinline__ gp<T> gstore(T* p) {
 
 // Lanes with `has` whose `start` lies in [base, base + 64) mark that slot
 // with their lane id; every lane returns the last mark at or before its own
-// slot, or `carry` (the owner of the previous window's last slot).
+// slot, or `carry` (the owner of the previous window's last slot). `mark`
+// holds 128 slots: lanes that mark nothing write into the upper half, so the
+// store needs no branch.
 __device__ __forceinline__ int wave_owner(int* mark, bool has, int start, int base, int carry) {
   const int lane = threadIdx.x & (kWave - 1);
   mark[lane] = -1;
   wave_lds_order();
-  if (has && start >= base && start < base + kWave) mark[start - base] = lane;
+  const unsigned rel = (unsigned)(start - base);
+  mark[has && rel < (unsigned)kWave ? (int)rel : kWave + lane] = lane;
   wave_lds_order();
   const int m = mark[lane];
   wave_lds_order();
@@ -193,10 +196,10 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
   float* vals = reinterpret_cast<float*>(st_lds);
   int* cols = BUILD ? st_lds : st_lds + cap4;
   int* tags = st_lds + 2 * cap4;
-  int* mark = st_lds + (BUILD ? 1 : SEARCH ? 3 : 1) * cap4;
+  int* mark = st_lds + (BUILD ? 1 : SEARCH ? 3 : 1) * cap4;  // 2 kWave slots
   // Per A entry of the current chunk: B position of its product at chunk
   // offset 0, scale, and accumulator (or C) offset of its row.
-  int4* etab = reinterpret_cast<int4*>(mark + kWave);
+  int4* etab = reinterpret_cast<int4*>(mark + 2 * kWave);
   const int lane = threadIdx.x;
   constexpr int kSegLanes = kStreamMaxSegs / kWave;
   unsigned bad = 0;  // kStreamBad* bits of refused writes
@@ -309,7 +312,10 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
             chi = cols[pn - 1];
           }
         }
-        long long qc = __shfl(rq, 0);  // stream index of the chunk's first product
+        // Stream index of the chunk's first product (uniform: lane 0's row).
+        long long qc = (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(
+                                        (int)((unsigned long long)rq >> 32))) << 32 |
+                                    (unsigned)__builtin_amdgcn_readfirstlane((int)rq));
         int ec = 0;
         for (int c0 = 0; c0 < E; c0 += kWave) {
           // One lane per A entry of the chunk.
@@ -381,6 +387,9 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
               // last (the map has 256 entries of slack past the stream); lanes
               // without a product read B position 0.
               const gp<const unsigned short> mc = ident ? nullptr : (gp<const unsigned short>)(map + qa);
+            const gp<const char> mc_g = (gp<const char>)mc;
+            const gp<const char> b_val_g = (gp<const char>)B.val;
+            const gp<const char> b_ci_g = (gp<const char>)B.ci;
               // Identity rows: C position of the product at chunk offset t is
               // base + t, base = the row's C start minus the chunk offset of
               // the row's first product.
@@ -411,7 +420,7 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
                   step(k, pv, t, bi, av, base, nlen);
                   const int dst = base + t;
                   if (pv && (dst < 0 || dst >= c_nnz)) bad |= kStreamBadValue;
-                  else if (pv) c_val[dst] = acc_step(0.0f, av, B.val[bi]);
+                  else if (pv) __builtin_nontemporal_store(acc_step(0.0f, av, B.val[bi]), &c_val[dst]);
                 }
               } else {
                 // Windows k + 1 and k + 2 are in flight while window k is
@@ -425,15 +434,24 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
                   float av, bv;
                   unsigned x;
                 };
+                // Loads address a uniform base plus a 32-bit unsigned byte
+                // offset (global loads with a scalar base, no 64-bit adds).
                 auto issue = [&](int k) {
                   Win w;
                   int t, bi;
                   step(k, w.pv, t, bi, w.av, w.acc, w.nlen);
-                  const int bq = w.pv ? bi : 0;
-                  w.x = SEARCH ? (unsigned)B.ci[bq] : (unsigned)mc[(k << 6) + lane];
-                  w.bv = B.val[bq];
+                  const unsigned bq = w.pv ? (unsigned)bi * 4u : 0u;
+                  // The map streams through once per compute: a non-temporal
+                  // load, so it does not evict the B rows the L2 reuses.
+                  w.x = SEARCH ? (unsigned)*(gp<const int>)(b_ci_g + bq)
+                               : (unsigned)__builtin_nontemporal_load((gp<const unsigned short>)(
+                                     mc_g + (unsigned)((k << 6) + lane) * 2u));
+                  w.bv = *(gp<const float>)(b_val_g + bq);
                   return w;
                 };
+                // Positions beyond the walk's range (or, in one walk, past its
+                // accumulators: a map that does not match) are skipped.
+                const unsigned plim = (unsigned)pn;
                 auto apply = [&](const Win& w) {
                   if (SEARCH) {
                     const int col = (int)w.x;
@@ -443,10 +461,8 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
                     if (use && (at >= pn || cols[at] != col)) bad |= kStreamBadColumn;
                     search_apply(vals, tags, use && at < pn, at, w.av, w.bv);
                   } else {
-                    const int rel = (int)(w.x & kPosMask) - p0;
-                    const int at = w.acc + rel;
-                    const bool use = w.pv && rel >= 0 && at < pn;
-                    if (npass == 1 && w.pv && !use) bad |= kStreamBadLds;
+                    const int at = w.acc + (int)(w.x & kPosMask) - p0;
+                    const bool use = w.pv && (unsigned)at < plim;
                     stream_apply(vals, use, at, w.x >> kOrderShift, w.av, w.bv);
                   }
                 };
@@ -484,7 +500,7 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
             const int dst = __shfl(cs, r) + p0 + j0 + lane - __shfl(abase, r);
             if (j0 + lane < pn) {
               if (dst < 0 || dst >= c_nnz) bad |= kStreamBadValue;
-              else c_val[dst] = vals[j0 + lane];
+              else __builtin_nontemporal_store(vals[j0 + lane], &c_val[dst]);  // written once
             }
           }
           wave_lds_order();
@@ -532,7 +548,7 @@ __global__ __launch_bounds__(kBlock) void k_stream_rounds(unsigned short* __rest
 // then the marks and the entry table.
 int stream_lds_bytes(int mode, int capw) {
   const int arrays = mode == kStreamSearchPass ? 3 : 1;
-  return arrays * ((capw + 3) & ~3) * 4 + kWave * 4 + kWave * 16;
+  return arrays * ((capw + 3) & ~3) * 4 + 2 * kWave * 4 + kWave * 16;
 }
 
 int stream_waves_per_cu(int mode, int lds) {

@@ -263,6 +263,7 @@ def test_full_size_2048_pairs_across_int32_groups(gpu):
         ci = torch.full((max(nnz, 1),), -1, dtype=torch.int32, device="cuda")
         val = torch.full((max(nnz, 1),), -1, dtype=torch.int32, device="cuda").view(torch.float32)
         rp = torch.full((A.rows + 1,), -1, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
         cm = F.CsrMatrix(A.rows, B.cols, nnz, rp.data_ptr(), ci.data_ptr(), val.data_ptr(),
                          F.DTYPE_FP32, F.FORMAT_CSR, 0)
         F.check(d.compute(A, B, cm))
@@ -274,6 +275,9 @@ def test_full_size_2048_pairs_across_int32_groups(gpu):
     assert bt.groups() >= 2
     assert [int(x) for x in nnz_b] == [w[0] for w in want]
     ci, val, rps, st = out_arrays(nnz_b)
+    # torch fills on its own stream; the library's work runs on another
+    # (non-blocking) stream, so the fills complete first.
+    torch.cuda.synchronize()
     offs = np.concatenate([[0], np.cumsum(np.asarray(nnz_b, np.int64))])
 
     def check_all():
@@ -289,6 +293,7 @@ def test_full_size_2048_pairs_across_int32_groups(gpu):
         F.check(bt.compute(As, Bs, st))
     assert bt.graph_launches() >= 1
     val.view(torch.int32).fill_(-1)
+    torch.cuda.synchronize()
     F.check(bt.compute(As, Bs, st))  # a graph replay rewrites every value
     check_all()
     # Oracle on sampled pairs of both groups.
