@@ -411,6 +411,8 @@ rvsp_status_t prepare_stream(BatchState& bs, hipStream_t s) {
   };
   std::vector<Pick> picks;
   int max_rows = 0, max_group_rows = 0;
+  // The passes address B's columns and values by 32-bit byte offsets.
+  if (bs.b_nnz_off.back() > kStreamMaxB) return RVSP_SUCCESS;
   for (size_t gi = 0; gi < bs.groups.size() && (int)picks.size() + 2 <= kStreamMaxSegs; ++gi) {
     const rvsp_spgemm_descr* in = bs.groups[gi].inner;
     const PhasePlan& p = in->numeric;

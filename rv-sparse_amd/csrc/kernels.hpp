@@ -304,6 +304,9 @@ constexpr unsigned kStreamBadMap = 2u;     // map entry outside the stream
 constexpr unsigned kStreamBadLds = 4u;     // accumulator outside the sub-batch
 constexpr unsigned kStreamBadColumn = 8u;  // product column not in its C row
 constexpr int kStreamItem = 8192;
+// Stacked-B entries the passes serve: they address B's columns and values
+// by 32-bit byte offsets.
+constexpr long long kStreamMaxB = 1LL << 30;
 constexpr int kStreamMaxRow = 2048;
 // Accumulators per wave of the numeric pass; longer rows are walked once
 // per range of this many positions.

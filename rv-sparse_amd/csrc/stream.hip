@@ -134,24 +134,28 @@ __device__ __forceinline__ int wave_owner(int* mark, bool has, int start, int ba
 }
 
 // One step's adds into the accumulators, in the rounds of the map's order
-// bits (lanes of one round have distinct positions).
-__device__ __forceinline__ void stream_apply(float* vals, bool pv, int at, unsigned ord, float a,
-                                             float b) {
+// bits (lanes of one round have distinct positions). `prod` is the rounded
+// product a * b, taken by every lane before any branch so that the loads it
+// reads are consumed on every path (a load left pending past the loop's end
+// makes the compiler wait for all loads in flight there).
+__device__ __forceinline__ void stream_apply(float* vals, bool pv, int at, unsigned ord,
+                                             float prod) {
+#pragma clang fp contract(off)
   if (__ballot(pv && ord != 0) == 0) {
-    if (pv) vals[at] = acc_step(vals[at], a, b);
+    if (pv) vals[at] = vals[at] + prod;
     return;
   }
 #pragma unroll 1
   for (unsigned r = 0; r < kLaneOrder; ++r) {
     if (__ballot(pv && ord >= r) == 0) return;
-    if (pv && ord == r) vals[at] = acc_step(vals[at], a, b);
+    if (pv && ord == r) vals[at] = vals[at] + prod;
     wave_lds_order();
   }
   const int lane = threadIdx.x & (kWave - 1);
   unsigned long long left = __ballot(pv && ord == kLaneOrder);
   while (left) {
     const int l = __ffsll((long long)left) - 1;
-    if (lane == l) vals[at] = acc_step(vals[at], a, b);
+    if (lane == l) vals[at] = vals[at] + prod;
     wave_lds_order();
     left &= left - 1;
   }
@@ -427,7 +431,8 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
                 // applied. Their loads are unconditional, also past the last
                 // window (no products there; map entries in the slack), so no
                 // branch makes the compiler wait for them early.
-                // x: the product's map entry (map pass) or B column (search).
+                // x: the aligned map dword holding the product's entry (map
+                // pass; the entry is its half msel) or B column (search).
                 struct Win {
                   bool pv;
                   int acc, nlen;
@@ -436,16 +441,23 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
                 };
                 // Loads address a uniform base plus a 32-bit unsigned byte
                 // offset (global loads with a scalar base, no 64-bit adds).
+                // Byte offset of the lane's map dword within a window, and
+                // the shift of its entry in that dword.
+                const unsigned mlane = (unsigned)(lane & ~1) * 2u;
+                const unsigned msel = (unsigned)(lane & 1) * 16u;
                 auto issue = [&](int k) {
                   Win w;
                   int t, bi;
                   step(k, w.pv, t, bi, w.av, w.acc, w.nlen);
                   const unsigned bq = w.pv ? (unsigned)bi * 4u : 0u;
                   // The map streams through once per compute: a non-temporal
-                  // load, so it does not evict the B rows the L2 reuses.
+                  // load, so it does not evict the B rows the L2 reuses. Two
+                  // lanes read the dword of their two entries: a 32-bit value
+                  // carried to the window's apply as loaded (a 16-bit one
+                  // is widened where the loop repeats, which waits for it).
                   w.x = SEARCH ? (unsigned)*(gp<const int>)(b_ci_g + bq)
-                               : (unsigned)__builtin_nontemporal_load((gp<const unsigned short>)(
-                                     mc_g + (unsigned)((k << 6) + lane) * 2u));
+                               : __builtin_nontemporal_load((gp<const unsigned>)(
+                                     mc_g + ((unsigned)k << 7) + mlane));
                   w.bv = *(gp<const float>)(b_val_g + bq);
                   return w;
                 };
@@ -453,6 +465,7 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
                 // accumulators: a map that does not match) are skipped.
                 const unsigned plim = (unsigned)pn;
                 auto apply = [&](const Win& w) {
+#pragma clang fp contract(off)
                   if (SEARCH) {
                     const int col = (int)w.x;
                     const bool use = w.pv && (npass == 1 || (col >= clo && col <= chi));
@@ -461,28 +474,34 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
                     if (use && (at >= pn || cols[at] != col)) bad |= kStreamBadColumn;
                     search_apply(vals, tags, use && at < pn, at, w.av, w.bv);
                   } else {
-                    const int at = w.acc + (int)(w.x & kPosMask) - p0;
+                    const unsigned ent = w.x >> msel;
+                    const int at = w.acc + (int)(ent & kPosMask) - p0;
                     const bool use = w.pv && (unsigned)at < plim;
-                    stream_apply(vals, use, at, w.x >> kOrderShift, w.av, w.bv);
+                    stream_apply(vals, use, at, (ent >> kOrderShift) & kLaneOrder, w.av * w.bv);
                   }
                 };
                 // Unrolled by three so the windows' registers change roles
                 // without copies (a copy of a load's destination would wait
-                // for the load).
+                // for the load). The loop has one exit, at its end: with
+                // exits between the windows, the compiler sees a path to the
+                // loop's start with a window's loads unused and waits there
+                // for every load in flight.
                 Win wa = issue(0);
                 Win wb = issue(1);
                 Win wc;
-                for (int k = 0;; k += 3) {
+                int k = 0;
+                for (; k + 3 <= nwin; k += 3) {
                   wc = issue(k + 2);
                   apply(wa);
-                  if (k + 1 >= nwin) break;
                   wa = issue(k + 3);
                   apply(wb);
-                  if (k + 2 >= nwin) break;
                   wb = issue(k + 4);
                   apply(wc);
-                  if (k + 3 >= nwin) break;
                 }
+                // Windows k and k + 1 are in flight; one past the chunk's
+                // last has no products and adds nothing.
+                apply(wa);
+                apply(wb);
               }
               wave_lds_order();  // the entry table is rewritten by the next chunk
             }
@@ -490,17 +509,23 @@ __global__ __launch_bounds__(kWave) void k_stream(const StreamSeg* __restrict__ 
           qc += T;
         }
         if (!BUILD && !ident) {
-          // Accumulators to C, row by row in order (coalesced runs); a walk
-          // of one long row writes its range.
+          // Accumulators to C in runs of rows that follow each other in C
+          // (their accumulators do too), the run's bounds uniform; a walk of
+          // one long row writes its range. Every store covers one 64-entry
+          // aligned block of C, so only a run's two ends write part of a
+          // cache line.
           wave_lds_order();
-          int rc = 0;
-          for (int j0 = 0; j0 < pn; j0 += kWave) {
-            const int r = wave_owner(mark, rl && n > 0, abase, j0, rc);
-            rc = wave_last(r);
-            const int dst = __shfl(cs, r) + p0 + j0 + lane - __shfl(abase, r);
-            if (j0 + lane < pn) {
+          for (int r = 0; r < R;) {
+            const int r_acc = __builtin_amdgcn_readlane(abase, r);
+            const int r_dst = __builtin_amdgcn_readlane(cs, r) + p0;
+            int r_n = npass > 1 ? pn : __builtin_amdgcn_readlane(n, r);
+            for (++r; r < R && __builtin_amdgcn_readlane(cs, r) == r_dst + r_n; ++r)
+              r_n += __builtin_amdgcn_readlane(n, r);
+            for (int j = lane - (r_dst & (kWave - 1)); j < r_n; j += kWave) {
+              if (j < 0) continue;
+              const int dst = r_dst + j;
               if (dst < 0 || dst >= c_nnz) bad |= kStreamBadValue;
-              else __builtin_nontemporal_store(vals[j0 + lane], &c_val[dst]);  // written once
+              else __builtin_nontemporal_store(vals[r_acc + j], &c_val[dst]);  // written once
             }
           }
           wave_lds_order();
