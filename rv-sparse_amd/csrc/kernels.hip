@@ -1784,7 +1784,6 @@ __device__ __forceinline__ int gm_gallop(const int* __restrict__ b_ci, int lo, i
 // kBlock entries), and split tables bound the searches of long B rows.
 constexpr int kTiledItems = 2;             // products per lane per group
 constexpr int kTiledGroup = kTiledItems * kBlock;
-constexpr int kTiledDepth = 2;             // groups whose gathers are in flight
 constexpr int kTiledWaves = kBlock / kWave;
 constexpr int kTagSlots = 256;             // per-wave hash of pending positions
 constexpr int kEntrySlots = 1024;          // entry table of a chunk
@@ -1819,6 +1818,7 @@ __device__ __forceinline__ TiledLds tiled_lds(int* lds, int T) {
 }
 
 constexpr int kCurChunks = 16;
+constexpr int kProbe = 8;  // B columns read together past a narrowed range's start
 
 // The A entry of product q of a chunk (the last e with pre[e] <= q): the
 // entry table gives the entry of the slot's first product, the kPrePad next
@@ -1941,27 +1941,47 @@ __device__ __forceinline__ void tiled_row(const TiledLds& L, int* scan_tmp, int 
           int be = B.rp[k + 1];
           if (narrow) {
             const int b0 = B.rp[k];
-            // A long B row's split table bounds both searches to one granule.
+            // A long B row's split table bounds both searches to one granule
+            // (read only when a search needs it).
             int so = -1;
-            if (split.off != nullptr && be - b0 >= kSplitMin && split.off[k + 1] > split.off[k])
-              so = split.off[k];
             int2 meta = make_int2(0, 0);
-            if (so >= 0) meta = *reinterpret_cast<const int2*>(split.sp + so);
+            auto load_split = [&] {
+              if (split.off != nullptr && be - b0 >= kSplitMin && split.off[k + 1] > split.off[k]) {
+                so = split.off[k];
+                meta = *reinterpret_cast<const int2*>(split.sp + so);
+              }
+            };
+            const bool resume = cursors && !first;
             int lo = 0;
-            if (cursors && !first) {
+            if (resume) {
 #pragma unroll
               for (int c = 0; c < kCurChunks; ++c)
                 if (c == chunk) lo = cur[c];
-            } else if (so >= 0) {
-              lo = split_lower_bound(B.ci, split.sp + so, meta, b0, be, b0, clo);
             } else {
-              lo = gm_lower_bound(B.ci, b0, be, clo);
+              load_split();
+              lo = so >= 0 ? split_lower_bound(B.ci, split.sp + so, meta, b0, be, b0, clo)
+                           : gm_lower_bound(B.ci, b0, be, clo);
             }
             bs = lo;
-            if (so >= 0)
-              be = split_lower_bound(B.ci, split.sp + so, meta, b0, be, lo, chi + 1);
-            else
-              be = gm_gallop(B.ci, lo, be, chi + 1);
+            // Most narrowed ranges hold a product or two: the columns at lo ..
+            // lo + kProbe - 1, read together (clamped to the row), end the
+            // range without a search unless all of them lie in the tile.
+            int n_in = 0;
+            if (lo < be) {
+#pragma unroll
+              for (int j = 0; j < kProbe; ++j) {
+                const int cj = B.ci[min(lo + j, be - 1)];
+                n_in += (lo + j < be && cj <= chi) ? 1 : 0;
+              }
+            }
+            if (n_in < kProbe) {
+              be = lo + n_in;
+            } else {
+              if (resume) load_split();
+              be = so >= 0 ? split_lower_bound(B.ci, split.sp + so, meta, b0, be, lo + kProbe,
+                                               chi + 1)
+                           : gm_gallop(B.ci, lo + kProbe, be, chi + 1);
+            }
             if (cursors) {
 #pragma unroll
               for (int c = 0; c < kCurChunks; ++c)
@@ -1987,60 +2007,52 @@ __device__ __forceinline__ void tiled_row(const TiledLds& L, int* scan_tmp, int 
         }
         __syncthreads();
         // Groups of kTiledGroup products, kTiledItems per lane (products
-        // q0 + i kBlock + tid), software-pipelined: the gathers of the next
-        // kTiledDepth groups are issued before this group's lookups and adds
-        // (occupancy is bounded by LDS here, not registers).
-        int n_col[kTiledDepth][kTiledItems];
-        float n_b[kTiledDepth][kTiledItems], n_a[kTiledDepth][kTiledItems];
-        auto gather = [&](int q, int& c, float& bv_, float& av_) {
+        // g kTiledGroup + i kBlock + tid), software-pipelined: the gathers of
+        // the next two groups are in flight while a group's lookups and adds
+        // run (occupancy is bounded by LDS here, not registers). Unrolled by
+        // three so the groups' registers change roles without copies (a copy
+        // of a load's destination waits for the load), with the loop's only
+        // exit at its end (an exit between the groups would leave a path to
+        // the loop's start with a group's gathers unused, where the compiler
+        // then waits for every load in flight).
+        // The gathers are unconditional (a lane past the chunk's last
+        // product reads that product again and drops it), and every group's
+        // loaded values are used on every path: a load skipped or left unused
+        // on some path makes the compiler wait there for later ones.
+        struct TGroup {
+          int col[kTiledItems];
+          float b[kTiledItems], a[kTiledItems];
+          bool ok[kTiledItems];
+        };
+        auto gather = [&](int q, int& c, float& bv_, float& av_, bool& ok) {
           // The A entry of product q (entries without products share their
           // successor's prefix).
-          c = -1;
-          bv_ = av_ = 0.0f;
-          if (q < total) {
-            const int e = tiled_entry(L, shift, q);
-            const int bi = L.bstart[e] + (q - L.pre[e]);
-            c = B.ci[bi];
-            bv_ = B.val[bi];
-            av_ = L.aval[e];
-          }
+          ok = q < total;
+          const int qq = ok ? q : total - 1;
+          const int e = tiled_entry(L, shift, qq);
+          const int bi = L.bstart[e] + (qq - L.pre[e]);
+          c = B.ci[bi];
+          bv_ = B.val[bi];
+          av_ = L.aval[e];
         };
-#pragma unroll
-        for (int d = 0; d < kTiledDepth; ++d)
-#pragma unroll
-          for (int i = 0; i < kTiledItems; ++i)
-            gather(d * kTiledGroup + i * kBlock + tid, n_col[d][i], n_b[d][i], n_a[d][i]);
-        for (int q0 = 0; q0 < total; q0 += kTiledGroup) {
-          int col[kTiledItems];
-          float bv[kTiledItems], av[kTiledItems];
-#pragma unroll
-          for (int i = 0; i < kTiledItems; ++i) {
-            col[i] = n_col[0][i];
-            bv[i] = n_b[0][i];
-            av[i] = n_a[0][i];
-          }
-#pragma unroll
-          for (int d = 0; d + 1 < kTiledDepth; ++d)
-#pragma unroll
-            for (int i = 0; i < kTiledItems; ++i) {
-              n_col[d][i] = n_col[d + 1][i];
-              n_b[d][i] = n_b[d + 1][i];
-              n_a[d][i] = n_a[d + 1][i];
-            }
+        auto issue = [&](int g) {
+          TGroup t;
 #pragma unroll
           for (int i = 0; i < kTiledItems; ++i)
-            gather(q0 + kTiledDepth * kTiledGroup + i * kBlock + tid, n_col[kTiledDepth - 1][i],
-                   n_b[kTiledDepth - 1][i], n_a[kTiledDepth - 1][i]);
+            gather(g * kTiledGroup + i * kBlock + tid, t.col[i], t.b[i], t.a[i], t.ok[i]);
+          return t;
+        };
+        auto process = [&](const TGroup& t) {
           int pos[kTiledItems], own[kTiledItems], rank[kTiledItems];
           float prod[kTiledItems];
 #pragma unroll
           for (int i = 0; i < kTiledItems; ++i) {
+            const bool in = t.ok[i] && t.col[i] >= clo && t.col[i] <= chi;
+            prod[i] = in ? t.a[i] * t.b[i] : 0.0f;  // rounded here, added below
             pos[i] = -1;
-            prod[i] = 0.0f;
             own[i] = kTiledWaves;  // no product
-            if (col[i] >= clo && col[i] <= chi) {  // col -1: no product
-              pos[i] = use_idx ? tiled_index_pos(L.idx, clo, col[i]) : lds_find(cols, tn, col[i]);
-              prod[i] = av[i] * bv[i];  // rounded here, added below
+            if (in) {
+              pos[i] = use_idx ? tiled_index_pos(L.idx, clo, t.col[i]) : lds_find(cols, tn, t.col[i]);
               own[i] = (pos[i] >> 5) & (kTiledWaves - 1);
             }
           }
@@ -2112,6 +2124,23 @@ __device__ __forceinline__ void tiled_row(const TiledLds& L, int* scan_tmp, int 
               }
             }
           }
+        };
+        const int ngroups = (total + kTiledGroup - 1) / kTiledGroup;  // block-uniform
+        if (ngroups > 0) {
+          TGroup ga = issue(0);
+          TGroup gb = issue(1);
+          TGroup gc;
+          int g = 0;
+          for (; g + 3 <= ngroups; g += 3) {
+            gc = issue(g + 2);
+            process(ga);
+            ga = issue(g + 3);
+            process(gb);
+            gb = issue(g + 4);
+            process(gc);
+          }
+          // The last one or two groups.
+          for (int r = g; r < ngroups; ++r) process(r == g ? ga : gb);
         }
         __syncthreads();  // the chunk's metadata is replaced next
       }
@@ -2273,25 +2302,38 @@ __global__ __launch_bounds__(kWave) void k_numeric_wave(
           if (narrow) {
             const int b0 = B.rp[k];
             int so = -1;
-            if (split.off != nullptr && be - b0 >= kSplitMin && split.off[k + 1] > split.off[k])
-              so = split.off[k];
             int2 meta = make_int2(0, 0);
-            if (so >= 0) meta = *reinterpret_cast<const int2*>(split.sp + so);
+            const bool resume = cursors && !first;
             int lo = 0;
-            if (cursors && !first) {
+            if (resume) {
 #pragma unroll
               for (int c = 0; c < kWaveCur; ++c)
                 if (c == chunk) lo = cur[c];
-            } else if (so >= 0) {
-              lo = split_lower_bound(B.ci, split.sp + so, meta, b0, be, b0, clo);
             } else {
-              lo = gm_lower_bound(B.ci, b0, be, clo);
+              if (split.off != nullptr && be - b0 >= kSplitMin && split.off[k + 1] > split.off[k]) {
+                so = split.off[k];
+                meta = *reinterpret_cast<const int2*>(split.sp + so);
+              }
+              lo = so >= 0 ? split_lower_bound(B.ci, split.sp + so, meta, b0, be, b0, clo)
+                           : gm_lower_bound(B.ci, b0, be, clo);
             }
             bs = lo;
-            if (so >= 0 && !(cursors && !first))
-              be = split_lower_bound(B.ci, split.sp + so, meta, b0, be, lo, chi + 1);
+            // The columns at lo .. lo + kProbe - 1, read together, end most
+            // ranges without a search.
+            int n_in = 0;
+            if (lo < be) {
+#pragma unroll
+              for (int j = 0; j < kProbe; ++j) {
+                const int cj = B.ci[min(lo + j, be - 1)];
+                n_in += (lo + j < be && cj <= chi) ? 1 : 0;
+              }
+            }
+            if (n_in < kProbe)
+              be = lo + n_in;
+            else if (so >= 0)
+              be = split_lower_bound(B.ci, split.sp + so, meta, b0, be, lo + kProbe, chi + 1);
             else
-              be = gm_gallop(B.ci, lo, be, chi + 1);
+              be = gm_gallop(B.ci, lo + kProbe, be, chi + 1);
             if (cursors) {
 #pragma unroll
               for (int c = 0; c < kWaveCur; ++c)
@@ -2429,7 +2471,7 @@ hipError_t launch_numeric_tiled_fused(const TiledSegs& segs, int T, int max_row,
                                       hipStream_t s) {
   if (segs.n <= 0 || segs.total <= 0) return hipSuccess;
   const long long most = max_row > 0 ? max_row : 1;
-  const int ytiles = (int)std::min<long long>(16, (most + T - 1) / T);
+  const int ytiles = (int)std::min<long long>(kTiledRuns, (most + T - 1) / T);
   const int gx = (std::min(segs.total, 1 << 20) + 7) / 8 * 8;
   hipLaunchKernelGGL(k_numeric_tiled_fused, dim3(gx, ytiles), dim3(kBlock), lds_bytes, s, segs,
                      T, A, B, split);
@@ -2460,7 +2502,7 @@ hipError_t launch_numeric_bin(const BinSpec& b, const int* rows, int nrows,
       // Tiles of one row get the same x, so they share an XCD (and its L2)
       // when the x extent is a multiple of the XCD count.
       const long long most = max_row > 0 ? max_row : 1;
-      const int ytiles = (int)std::min<long long>(16, (most + b.cap - 1) / b.cap);
+      const int ytiles = (int)std::min<long long>(kTiledRuns, (most + b.cap - 1) / b.cap);
       const int gx = (std::min(nrows, 1 << 20) + 7) / 8 * 8;
       if (b.variant == 1) {
         // Wave tiles: up to kWaveRuns runs per row, one wave each.

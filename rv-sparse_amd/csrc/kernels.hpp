@@ -223,6 +223,9 @@ hipError_t launch_numeric_tiled_fused(const TiledSegs& segs, int T, int max_row,
                                       hipStream_t s);
 // Default tile of the long-row numeric kernel: three workgroups fit a CU.
 constexpr int kTiledPositions = 6144;
+// Runs (workgroups) a long row is cut into at most (more runs measured
+// slower on R-MAT rows: every run walks the row's A entries once per tile).
+constexpr int kTiledRuns = 16;
 // Dynamic LDS of the wave-tile long-row numeric kernel (KIND_BLOCK variant
 // 1) for tiles of Tw positions (a power of two, 256..2048); a workgroup is
 // one wave, and a row gets up to kWaveRuns runs of tiles.
