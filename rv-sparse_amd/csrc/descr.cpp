@@ -220,6 +220,7 @@ void drop_structure(rvsp_spgemm_descr* d) {
   d->map_bins = 0;
   d->map_width = 0;
   d->map_promoted = false;
+  d->brow_ok = false;
   d->split_valid = false;
   d->keep_ok = false;
   d->keep_bins = 0;
@@ -658,10 +659,7 @@ rvsp_status_t run_numeric(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B
     }
     if (spec.kind == KIND_MAP) {
       RVSP_TRY(ensure_map_built(d, A, B, c_ci, s));
-      MapView mv;
-      mv.map = d->map.p;
-      mv.poff = (const long long*)d->poff.p;
-      mv.width = d->map_width;
+      MapView mv = map_view(d);
       mv.row_base = numeric_row_base(d, rows);
       RVSP_HIP_TRY(launch_numeric_map_bin(spec, mv, rows, n, A, B, (const int*)d->c_rp.p,
                                           c_val, s));
@@ -672,6 +670,18 @@ rvsp_status_t run_numeric(rvsp_spgemm_descr* d, const DevCsr& A, const DevCsr& B
     }
   }
   return RVSP_SUCCESS;
+}
+
+MapView map_view(const rvsp_spgemm_descr* d) {
+  MapView mv;
+  mv.map = d->map.p;
+  mv.poff = (const long long*)d->poff.p;
+  mv.width = d->map_width;
+  if (d->brow_ok) {
+    mv.brow = (const int*)d->brow.p;
+    mv.blen = (const unsigned short*)((const char*)d->brow.p + (size_t)d->a_nnz * 4);
+  }
+  return mv;
 }
 
 int numeric_row_base(const rvsp_spgemm_descr* d, const int* rows) {
@@ -694,6 +704,7 @@ rvsp_status_t plan_map(rvsp_spgemm_descr* d, hipStream_t s) {
   d->map_valid = false;
   d->map_bins = 0;
   d->map_width = 0;
+  d->brow_ok = false;
   if (mask == 0) return RVSP_SUCCESS;
   const int M = p.offsets.empty() ? 0 : p.offsets.back();  // rows partitioned
   const int width = max_bound <= 256 ? 1 : 2;  // positions are < the row's nnz
@@ -722,6 +733,12 @@ rvsp_status_t plan_map(rvsp_spgemm_descr* d, hipStream_t s) {
   }
   d->map_bins = mask;
   d->map_width = width;
+  // The B-row cache: a B row a map row reaches is no longer than the row's
+  // product count (at most max_bound), so 16-bit lengths hold it.
+  const size_t brow_bytes = (size_t)d->a_nnz * 6 + 16;
+  if (max_bound <= 65535 && d->a_nnz > 0 && bytes + brow_bytes <= budget &&
+      ensure(d->brow, brow_bytes) == RVSP_SUCCESS)
+    d->brow_ok = true;  // filled with the map entries
   return RVSP_SUCCESS;
 }
 
@@ -735,6 +752,11 @@ rvsp_status_t ensure_map_built(rvsp_spgemm_descr* d, const DevCsr& A, const DevC
     RVSP_HIP_TRY(launch_build_map(d->map_width, (const int*)p.rows.p + p.offsets[b],
                                   bin_rows(p, b), A, B, (const int*)d->c_rp.p, c_ci,
                                   (const long long*)d->poff.p, d->map.p, s));
+  }
+  if (d->brow_ok) {
+    const MapView mv = map_view(d);
+    RVSP_HIP_TRY(launch_brow_cache(A, B, const_cast<int*>(mv.brow),
+                                   const_cast<unsigned short*>(mv.blen), s));
   }
   d->map_valid = true;
   return RVSP_SUCCESS;
@@ -751,10 +773,12 @@ rvsp_status_t release_position_map(rvsp_spgemm_descr* d) {
   release(d->map);
   release(d->poff);
   release(d->map_weights);
+  release(d->brow);
   d->map_valid = false;
   d->map_bins = 0;
   d->map_width = 0;
   d->map_promoted = false;
+  d->brow_ok = false;
   return RVSP_SUCCESS;
 }
 
@@ -822,7 +846,7 @@ void rvsp_spgemm_descr_destroy(rvsp_spgemm_descr_t descr) {
                     &d->tile_hist, &d->tile_base, &d->bin_offsets, &d->count.rows,
                     &d->fill.rows, &d->numeric.rows, &d->c_ci_dev, &d->c_val_dev,
                     &d->gbits, &d->gdense, &d->learn_feat, &d->bin_stats,
-                    &d->map, &d->poff, &d->map_weights, &d->keep, &d->koff,
+                    &d->map, &d->poff, &d->map_weights, &d->brow, &d->keep, &d->koff,
                     &d->keep_w, &d->split_cnt, &d->split_off, &d->split_sp,
                     &d->check_key};
   // The streams are drained above: the buffers go back to the cache idle.

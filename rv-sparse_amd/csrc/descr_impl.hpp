@@ -114,6 +114,10 @@ struct rvsp_spgemm_descr {
   bool map_valid = false;   // entries built for the current structure
   bool map_promoted = false;  // searching bins already moved to the map
   bool position_map = true;   // promote on the first numeric-only compute
+  // B-row cache of the map kernels (per A entry: B-row start, 16-bit
+  // length), built with the map entries; brow_ok: usable by the map bins.
+  rvsp::DevBuf brow;
+  bool brow_ok = false;
 
   // Split table of B's long rows for the tiled numeric kernel's narrowing
   // searches (per-row counts, offsets, table).
@@ -220,6 +224,10 @@ int tuned_rule(int phase, int key);
 
 // Rows of bin b of a partitioned phase.
 inline int bin_rows(const PhasePlan& p, int b) { return p.offsets[b + 1] - p.offsets[b]; }
+
+// The descriptor's position map (and B-row cache, when built) as the map
+// kernels take it; row_base left at -1.
+MapView map_view(const rvsp_spgemm_descr* d);
 
 // row_base of the numeric bin whose row list starts at `rows` (-1 if none).
 int numeric_row_base(const rvsp_spgemm_descr* d, const int* rows);

@@ -1330,6 +1330,29 @@ __global__ __launch_bounds__(kBlock) void k_build_map(const int* __restrict__ ro
   }
 }
 
+// Entry A.rp[0] + i of A (i < nnz) at slot i: A's entries may start past 0
+// (a row range of a larger matrix).
+__global__ __launch_bounds__(kBlock) void k_brow_cache(const int* __restrict__ a_rp,
+                                                       const int* __restrict__ a_ci, int nnz,
+                                                       const int* __restrict__ b_rp,
+                                                       int* __restrict__ brow,
+                                                       unsigned short* __restrict__ blen) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nnz) return;
+  const int k = a_ci[a_rp[0] + i];
+  const int bs = b_rp[k];
+  brow[i] = bs;
+  blen[i] = (unsigned short)min(b_rp[k + 1] - bs, 65535);
+}
+
+hipError_t launch_brow_cache(const DevCsr& A, const DevCsr& B, int* brow, unsigned short* blen,
+                             hipStream_t s) {
+  if (A.nnz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_brow_cache, dim3((A.nnz + kBlock - 1) / kBlock), dim3(kBlock), 0, s, A.rp,
+                     A.ci, A.nnz, B.rp, brow, blen);
+  return hipGetLastError();
+}
+
 hipError_t launch_build_map(int width, const int* rows, int nrows, const DevCsr& A,
                             const DevCsr& B, const int* c_rp, const int* c_ci,
                             const long long* poff, void* map, hipStream_t s) {
@@ -1356,7 +1379,17 @@ template <int G, typename T, int VAR>
 __global__ __launch_bounds__(kBlock) void k_numeric_map(
     const int* __restrict__ rows, int nrows, int row_base, int cap, DevCsr A, DevCsr B,
     const int* __restrict__ c_rp, const long long* __restrict__ poff,
-    const T* __restrict__ map, float* __restrict__ c_val) {
+    const T* __restrict__ map, float* __restrict__ c_val, const int* __restrict__ brow,
+    const unsigned short* __restrict__ blen) {
+  // With the B-row cache, an entry's B row comes from brow/blen at its own
+  // position relative to A's first (no load of A's column, then of B's row
+  // pointers).
+  const bool cached = brow != nullptr;  // uniform
+  if (cached) {
+    const int a_base = A.rp[0];
+    brow -= a_base;
+    blen -= a_base;
+  }
   extern __shared__ int lds[];
   constexpr int kGroups = kBlock / G;
   constexpr int kMapCap = kGroups * kSegMapPerRow;  // map entries staged per segment
@@ -1430,10 +1463,15 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
     if (a_staged) {
       const int na = A.rp[last + 1] - a_off;
       for (int i = threadIdx.x; i < na; i += kBlock) {
-        const int k = A.ci[a_off + i];
         m_a[i] = A.val[a_off + i];
-        m_bs[i] = B.rp[k];
-        m_be[i] = B.rp[k + 1];
+        if (cached) {
+          m_bs[i] = brow[a_off + i];
+          m_be[i] = m_bs[i] + blen[a_off + i];
+        } else {
+          const int k = A.ci[a_off + i];
+          m_bs[i] = B.rp[k];
+          m_be[i] = B.rp[k + 1];
+        }
       }
     }
     __syncthreads();
@@ -1471,6 +1509,10 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
       *a = m_a[e - a_off];
       *bs = m_bs[e - a_off];
       *be = m_be[e - a_off];
+    } else if (cached) {
+      *a = A.val[e];
+      *bs = brow[e];
+      *be = *bs + blen[e];
     } else {
       const int k = A.ci[e];
       *a = A.val[e];
@@ -1500,16 +1542,26 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
           constexpr int kPer = (kChunk + G - 1) / G;
           int kl[kPer], bl[kPer], el[kPer];
           float al[kPer];
+          if (cached) {
 #pragma unroll
-          for (int u = 0; u < kPer; ++u) {
-            const int e = min(c0 + l + u * G, a_e - 1);
-            kl[u] = A.ci[e];
-            al[u] = A.val[e];
-          }
+            for (int u = 0; u < kPer; ++u) {
+              const int e = min(c0 + l + u * G, a_e - 1);
+              al[u] = A.val[e];
+              bl[u] = brow[e];
+              el[u] = bl[u] + blen[e];
+            }
+          } else {
 #pragma unroll
-          for (int u = 0; u < kPer; ++u) {
-            bl[u] = B.rp[kl[u]];
-            el[u] = B.rp[kl[u] + 1];
+            for (int u = 0; u < kPer; ++u) {
+              const int e = min(c0 + l + u * G, a_e - 1);
+              kl[u] = A.ci[e];
+              al[u] = A.val[e];
+            }
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+              bl[u] = B.rp[kl[u]];
+              el[u] = B.rp[kl[u] + 1];
+            }
           }
 #pragma unroll
           for (int u = 0; u < kChunk; ++u) {
@@ -1620,12 +1672,13 @@ template <int G>
 static hipError_t launch_numeric_map_g(const BinSpec& b, int width, const int* rows, int nrows,
                                        int row_base, const DevCsr& A, const DevCsr& B,
                                        const int* c_rp, const long long* poff, const void* map,
-                                       float* c_val, hipStream_t s) {
+                                       float* c_val, const int* brow, const unsigned short* blen,
+                                       hipStream_t s) {
   const int groups = kBlock / G;
   const int grid = (nrows + groups - 1) / groups;
 #define RVSP_MAP_LAUNCH(T, SA)                                                              \
   hipLaunchKernelGGL((k_numeric_map<G, T, SA>), dim3(grid), dim3(kBlock), b.lds_bytes, s, rows, \
-                     nrows, row_base, b.cap, A, B, c_rp, poff, (const T*)map, c_val)
+                     nrows, row_base, b.cap, A, B, c_rp, poff, (const T*)map, c_val, brow, blen)
   if (b.variant < 0 || b.variant > 2) return hipErrorInvalidValue;
   if (width == 1) {
     if (b.variant == 0) RVSP_MAP_LAUNCH(unsigned char, 0);
@@ -1646,13 +1699,13 @@ hipError_t launch_numeric_map_bin(const BinSpec& b, const MapView& mv, const int
   if (nrows <= 0) return hipSuccess;
   if (!mv.map || !mv.poff || (mv.width != 1 && mv.width != 2)) return hipErrorInvalidValue;
   switch (b.group) {
-    case 1: return launch_numeric_map_g<1>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, s);
-    case 2: return launch_numeric_map_g<2>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, s);
-    case 4: return launch_numeric_map_g<4>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, s);
-    case 8: return launch_numeric_map_g<8>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, s);
-    case 16: return launch_numeric_map_g<16>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, s);
-    case 32: return launch_numeric_map_g<32>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, s);
-    case 64: return launch_numeric_map_g<64>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, s);
+    case 1: return launch_numeric_map_g<1>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, mv.brow, mv.blen, s);
+    case 2: return launch_numeric_map_g<2>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, mv.brow, mv.blen, s);
+    case 4: return launch_numeric_map_g<4>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, mv.brow, mv.blen, s);
+    case 8: return launch_numeric_map_g<8>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, mv.brow, mv.blen, s);
+    case 16: return launch_numeric_map_g<16>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, mv.brow, mv.blen, s);
+    case 32: return launch_numeric_map_g<32>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, mv.brow, mv.blen, s);
+    case 64: return launch_numeric_map_g<64>(b, mv.width, rows, nrows, mv.row_base, A, B, c_rp, mv.poff, mv.map, c_val, mv.brow, mv.blen, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -239,7 +239,17 @@ struct MapView {
   const long long* poff = nullptr;  // rows + 1 entries
   int width = 0;                    // bytes per entry (1 or 2); 0 = no map
   int row_base = -1;  // >= 0: the listed rows are row_base, row_base + 1, ...
+  // Optional B-row cache (null: none): for A's entry e, slot e - A.rp[0]
+  // holds the first position (brow) and length (blen) of the B row that
+  // A's column at e names, so the map kernels read no column of A and no
+  // row pointer of B.
+  const int* brow = nullptr;
+  const unsigned short* blen = nullptr;
 };
+// Fills the B-row cache of A's nnz entries from A.rp[0] on (lengths capped
+// at 65535; only rows whose product count fits 16 bits may use it).
+hipError_t launch_brow_cache(const DevCsr& A, const DevCsr& B, int* brow, unsigned short* blen,
+                             hipStream_t s);
 // poff = exclusive scan over rows of (bin_of[i] in bin_mask ? Op_i : 0);
 // *total = poff[rows]. `weights` holds rows int64 values of scratch.
 // span_bound: weights min(Op_i, span_i) instead of Op_i.

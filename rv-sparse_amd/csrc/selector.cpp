@@ -162,10 +162,7 @@ hipError_t launch_bin(rvsp_spgemm_descr* d, int phase, const BinSpec& spec, cons
                              d->gs_bits, s);
     default:
       if (spec.kind == KIND_MAP) {
-        MapView mv;
-        mv.map = d->map.p;
-        mv.poff = (const long long*)d->poff.p;
-        mv.width = d->map_width;
+        MapView mv = map_view(d);
         mv.row_base = numeric_row_base(d, rows);
         return launch_numeric_map_bin(spec, mv, rows, n, A, B, (const int*)d->c_rp.p, c_val, s);
       }
