@@ -182,24 +182,52 @@ __device__ __forceinline__ void block_for_each_col_items(int a_s, int a_e,
     sh_bs[t] = bs;
     etab[t] = len;
     __syncthreads();
-    for (int it = wid; it < total; it += kWaves) {
-      // The entry of item `it`: the last e with sh_excl[e] <= it.
-      int lo = 0, n = NT;
-      while (n > 1) {
-        const int half = n >> 1;
-        lo = sh_excl[lo + half] <= it ? lo + half : lo;
-        n -= half;
-      }
-      const int eb = sh_bs[lo], el = etab[lo];
-      const int o0 = (it - sh_excl[lo]) * kItem;
-      int c[kItem / kWave];
+    // This wave's items wid, wid + kWaves, ...: each one's gathers are in
+    // flight while the previous one is applied. The gathers are
+    // unconditional (inside the entry's B row; a fetch past the wave's last
+    // item reads that item again and applies nothing), and every gathered
+    // column is used on every path, so the compiler waits only for the item
+    // being applied.
+    const int nmine = total > wid ? (total - wid + kWaves - 1) / kWaves : 0;  // wave-uniform
+    if (nmine > 0) {
+      struct Item {
+        int c[kItem / kWave];
+        int lim;  // valid positions of the item
+      };
+      auto fetch = [&](int j) {
+        Item x;
+        const int it = wid + min(j, nmine - 1) * kWaves;
+        // The entry of item `it`: the last e with sh_excl[e] <= it.
+        int lo = 0, n = NT;
+        while (n > 1) {
+          const int half = n >> 1;
+          lo = sh_excl[lo + half] <= it ? lo + half : lo;
+          n -= half;
+        }
+        const int eb = sh_bs[lo], el = etab[lo];  // el >= kItemMin
+        const int o0 = (it - sh_excl[lo]) * kItem;
+        x.lim = j < nmine ? el - o0 : 0;
 #pragma unroll
-      for (int u = 0; u < kItem / kWave; ++u) {
-        const int o = o0 + u * kWave + lane;
-        c[u] = o < el ? b_ci[eb + o] : -1;
-      }
+        for (int u = 0; u < kItem / kWave; ++u) x.c[u] = b_ci[eb + min(o0 + u * kWave + lane, el - 1)];
+        return x;
+      };
+      auto apply = [&](const Item& x) {
 #pragma unroll
-      for (int u = 0; u < kItem / kWave; ++u) f(c[u], c[u] >= 0);
+        for (int u = 0; u < kItem / kWave; ++u) {
+          const bool v = u * kWave + lane < x.lim;
+          f(v ? x.c[u] : -1, v);
+        }
+      };
+      Item xa = fetch(0);
+      Item xb;
+      int j = 0;
+      for (; j + 2 <= nmine; j += 2) {
+        xb = fetch(j + 1);
+        apply(xa);
+        xa = fetch(j + 2);
+        apply(xb);
+      }
+      if (j < nmine) apply(xa);
     }
     __syncthreads();
   }
