@@ -320,14 +320,20 @@ constexpr int kStreamItem = 8192;
 // Stacked-B entries the passes serve: they address B's columns and values
 // by 32-bit byte offsets.
 constexpr long long kStreamMaxB = 1LL << 30;
-constexpr int kStreamMaxRow = 2048;
+// Longest rows (products) the passes serve; longer ones stay with the
+// long-row kernels.
+constexpr int kStreamMaxRow = 4096;
 // Accumulators per wave of the numeric pass; longer rows are walked once
 // per range of this many positions.
 constexpr int kStreamNumericRow = 1024;
 // The same for the search pass (accumulators, columns and order tags).
 constexpr int kStreamSearchRow = 512;
 constexpr int kStreamMaxSegs = 256;
-constexpr int kStreamPosBits = 11;
+// Map entry (16 bits): position in the C row (bits 0-11, < kStreamMaxRow),
+// row mark (bit 12: the row's first product), order among the window's
+// earlier products of the same C entry (bits 13-15).
+constexpr int kStreamPosBits = 12;
+static_assert((1 << kStreamPosBits) >= kStreamMaxRow, "map positions must cover the rows");
 // w[i] = ops[row of list position i].
 hipError_t launch_stream_weights(const int* rows, int row_base, int nrows, const long long* ops,
                                  long long* w, hipStream_t s);

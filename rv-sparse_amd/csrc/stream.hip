@@ -35,8 +35,9 @@ namespace {
 
 constexpr unsigned kPosMask = (1u << kStreamPosBits) - 1u;
 constexpr unsigned kRowMark = 1u << kStreamPosBits;
-constexpr int kOrderShift = 12;
-constexpr unsigned kLaneOrder = 15;  // order value: apply in lane order
+constexpr int kOrderShift = kStreamPosBits + 1;
+constexpr unsigned kLaneOrder = 7;  // order value: apply in lane order
+static_assert(kOrderShift + 3 == 16, "the order takes the entry's top three bits");
 
 __global__ __launch_bounds__(kBlock) void k_stream_weights(const int* __restrict__ rows,
                                                            int row_base, int nrows,

@@ -4,7 +4,7 @@ product-balanced work queue. Every pair must stay bit-identical to the
 oracle, with the pass on and off, eager and replayed from the graph, with new
 values each time and outputs pre-filled with a sentinel, including rows whose
 products collide on the same C entry many times inside one 64-product window
-(order rounds, and the lane-order fallback past 15 repeats)."""
+(order rounds, and the lane-order fallback past 7 repeats)."""
 import numpy as np
 import pytest
 
@@ -32,11 +32,17 @@ def collision_pairs():
     mask = rng.random((300, n)) < rng.uniform(0.05, 0.9, (300, 1))
     A = Csr.from_dense((mask * rng.uniform(-1, 1, (300, n))).astype(np.float32))
     pairs.append((A, B))
-    # C rows of about 600 to 2600 positions (the stream's longest rows and,
-    # past 2048, rows left to the long-row kernels).
+    # C rows of about 600 to 2600 positions (the stream's long rows, walked
+    # once per range of accumulators).
     A2 = gen.lognormal_rows(3000, mu=np.log(150.0), sigma=0.5, cap=400, seed=12)
     B2 = gen.lognormal_rows(3000, mu=np.log(8.0), sigma=0.5, cap=16, seed=13)
     pairs.append((A2, B2))
+    # Mostly short rows and a tail of rows with up to about 10000 products
+    # and 5000 positions: past 4096 products, rows left to the long-row
+    # kernels, next to the stream's.
+    A4 = gen.lognormal_rows(6000, mu=np.log(20.0), sigma=1.2, cap=500, seed=17)
+    B4 = gen.lognormal_rows(6000, mu=np.log(20.0), sigma=0.3, cap=40, seed=18)
+    pairs.append((A4, B4))
     # Rows fed by one B row (identity positions) next to mixed rows.
     A3 = gen.uniform_k(1500, 1, seed=14)
     B3 = gen.uniform_k(1500, 30, seed=15)
