@@ -194,7 +194,7 @@ rvsp_status_t rvsp_spgemm_batch_set_graph(rvsp_spgemm_descr_t descr, int on);
  * bits are the same either way. Fusing saves launches on many small groups
  * but gives up the overlap. Must precede the batch estimation. */
 rvsp_status_t rvsp_spgemm_batch_set_fuse(rvsp_spgemm_descr_t descr, int on);
-/* How batch computes run the groups' numeric bins of rows up to 2048 entries
+/* How batch computes run the groups' numeric bins of rows up to 4096 entries
  * long (and of rows fed by a single B row): as one persistent launch over a
  * product-balanced work queue of (group, bin, row block) items, captured with
  * the rest of a numeric-only compute in the batch's graph; the first compute
