@@ -1404,7 +1404,7 @@ hipError_t launch_build_map(int width, const int* rows, int nrows, const DevCsr&
 // pointers at the cost of LDS and one more load round before the barrier
 // (VAR 1). VAR 2 instead fetches A entries in chunks (see below).
 template <int G, typename T, int VAR>
-__global__ __launch_bounds__(kBlock) void k_numeric_map(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(G <= 2 ? 8 : 1))) void k_numeric_map(
     const int* __restrict__ rows, int nrows, int row_base, int cap, DevCsr A, DevCsr B,
     const int* __restrict__ c_rp, const long long* __restrict__ poff,
     const T* __restrict__ map, float* __restrict__ c_val, const int* __restrict__ brow,
@@ -1423,9 +1423,12 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
   constexpr int kMapCap = kGroups * kSegMapPerRow;  // map entries staged per segment
   constexpr int kACap = kGroups * kSegAPerRow;      // A entries staged per segment
   constexpr int kU = VAR == 2 ? 3 : 2;
+  // Accumulators, then one 16-byte slot that the branch-free adds of lanes
+  // without a product go to (VAR 2), the staged map slice and A metadata.
   float* vals = reinterpret_cast<float*>(lds);
-  T* mstage = reinterpret_cast<T*>(lds + kGroups * cap);
-  float* m_a = reinterpret_cast<float*>(lds + kGroups * cap + map_stage_bytes(kGroups) / 4);
+  const int spare = kGroups * cap;  // that slot's index in vals
+  T* mstage = reinterpret_cast<T*>(lds + kGroups * cap + 4);
+  float* m_a = reinterpret_cast<float*>(lds + kGroups * cap + 4 + map_stage_bytes(kGroups) / 4);
   int* m_bs = reinterpret_cast<int*>(m_a + kACap);
   int* m_be = m_bs + kACap;
   __shared__ int s_crp[kGroups + 1], s_arp[kGroups + 1];
@@ -1560,8 +1563,10 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
     // and written back together; entry u + 1's reads follow entry u's writes
     // in program order.
     constexpr int kChunk = 5;
-    if (active && a_s < a_e) {
+    if (active && a_s < a_e && B.nnz > 0) {
       float* rv = vals + base;
+      const int rv_spare = spare - base;
+      const int bmax = B.nnz - 1;
       int t0 = 0;
       auto apply_chunk = [&](const auto* mp) {
         for (int c0 = a_s; c0 < a_e; c0 += kChunk) {
@@ -1598,28 +1603,52 @@ __global__ __launch_bounds__(kBlock) void k_numeric_map(
             bb[u] = __shfl(bl[u / G], u % G, G);
             ee[u] = h ? __shfl(el[u / G], u % G, G) : bb[u];
           }
+          // The first kU products of each entry. With G <= 2 branch-free
+          // (the kernel then fits 64 registers): every lane loads a B value
+          // (past its entry's end, a valid entry of B that is dropped) and a
+          // map entry (past its entry's end, the entry's last or, for an
+          // empty entry, the one after the row's products, within the map's
+          // slack), and adds into its position or, with no product, into the
+          // spare slot. Wider groups keep the per-product branches, which
+          // hold fewer registers.
 #pragma unroll
           for (int u = 0; u < kChunk; ++u)
 #pragma unroll
             for (int j = 0; j < kU; ++j) {
               const int p = bb[u] + l + j * G;
-              vv[u][j] = p < ee[u] ? B.val[p] : 0.0f;
+              if constexpr (G <= 2)
+                vv[u][j] = B.val[min(p, bmax)];
+              else
+                vv[u][j] = p < ee[u] ? B.val[p] : 0.0f;
             }
 #pragma unroll
           for (int u = 0; u < kChunk; ++u) {
             const int len = ee[u] - bb[u];  // 0 past the row's last entry
             int pos[kU];
             float cur[kU];
+            if constexpr (G <= 2) {
 #pragma unroll
-            for (int j = 0; j < kU; ++j) {
-              const int t = l + j * G;
-              pos[j] = t < len ? (int)mp[t0 + t] : -1;
+              for (int j = 0; j < kU; ++j) {
+                const int t = l + j * G;
+                const int m = (int)mp[t0 + min(t, max(len - 1, 0))];
+                pos[j] = t < len ? m : rv_spare;
+              }
+#pragma unroll
+              for (int j = 0; j < kU; ++j) cur[j] = rv[pos[j]];
+#pragma unroll
+              for (int j = 0; j < kU; ++j) rv[pos[j]] = acc_step(cur[j], aa[u], vv[u][j]);
+            } else {
+#pragma unroll
+              for (int j = 0; j < kU; ++j) {
+                const int t = l + j * G;
+                pos[j] = t < len ? (int)mp[t0 + t] : -1;
+              }
+#pragma unroll
+              for (int j = 0; j < kU; ++j) cur[j] = pos[j] >= 0 ? rv[pos[j]] : 0.0f;
+#pragma unroll
+              for (int j = 0; j < kU; ++j)
+                if (pos[j] >= 0) rv[pos[j]] = acc_step(cur[j], aa[u], vv[u][j]);
             }
-#pragma unroll
-            for (int j = 0; j < kU; ++j) cur[j] = pos[j] >= 0 ? rv[pos[j]] : 0.0f;
-#pragma unroll
-            for (int j = 0; j < kU; ++j)
-              if (pos[j] >= 0) rv[pos[j]] = acc_step(cur[j], aa[u], vv[u][j]);
             for (int t = kU * G + l; t < len; t += G) {
               const int q = mp[t0 + t];
               rv[q] = acc_step(rv[q], aa[u], B.val[bb[u] + t]);
