@@ -103,6 +103,11 @@ struct BatchState {
   std::vector<long long> a_row_off, a_nnz_off, b_row_off, b_nnz_off;
   int b_cols = 1;
   DevBuf a_rp, a_ci, a_val, b_rp, b_ci, b_val;
+  // Every pair multiplies a matrix by itself through the same value array
+  // (C = A * A): the stacked B values are the stacked A values, and only
+  // those are staged per compute.
+  bool b_val_shared = false;
+  const void* b_vals() const { return b_val_shared ? a_val.p : b_val.p; }
   DevBuf ops, gather_idx[kMaxLanes], gather_out[kMaxLanes];
   std::vector<BatchGroup> groups;
   std::vector<int32_t> c_nnz;
@@ -296,6 +301,14 @@ void build_value_table(BatchState& bs, const rvsp_csr_matrix_t* A, const rvsp_cs
   t.clear();
   for (int p = 0; p < bs.count; ++p)
     t.add((const int*)A[p].values, (int*)bs.a_val.p + bs.a_nnz_off[p], A[p].nnz, 0, 0);
+  // Stacked B j is pair j's B for every j, and each holds the values array
+  // of its pair's A (then also the same entry count and stack offsets).
+  bool shared = (int)bs.b_pair.size() == bs.count;
+  for (int p = 0; p < bs.count && shared; ++p)
+    shared = bs.b_pair[p] == p && bs.b_of[p] == p && B[p].values == A[p].values &&
+             B[p].nnz == A[p].nnz && bs.b_nnz_off[p] == bs.a_nnz_off[p];
+  bs.b_val_shared = shared;
+  if (shared) return;
   for (int j = 0; j < (int)bs.b_pair.size(); ++j) {
     const rvsp_csr_matrix_t& Bj = B[bs.b_pair[j]];
     t.add((const int*)Bj.values, (int*)bs.b_val.p + bs.b_nnz_off[j], Bj.nnz, 0, 0);
@@ -387,7 +400,7 @@ DevCsr stacked_b(const BatchState& bs) {
   B.nnz = (int)bs.b_nnz_off.back();
   B.rp = (const int*)bs.b_rp.p;
   B.ci = (const int*)bs.b_ci.p;
-  B.val = (const float*)bs.b_val.p;
+  B.val = (const float*)bs.b_vals();
   return B;
 }
 
@@ -704,7 +717,7 @@ rvsp_status_t launch_deferred(BatchState& bs, hipStream_t s) {
   B.nnz = (int)bs.b_nnz_off.back();
   B.rp = (const int*)bs.b_rp.p;
   B.ci = (const int*)bs.b_ci.p;
-  B.val = (const float*)bs.b_val.p;
+  B.val = (const float*)bs.b_vals();
   std::vector<bool> done(all.size(), false);
   for (size_t i = 0; i < all.size(); ++i) {
     if (done[i]) continue;
@@ -740,7 +753,7 @@ rvsp_status_t batch_body(BatchState& bs, const rvsp_csr_matrix_t* C, bool full_o
   for (BatchGroup& g : bs.groups) g.inner->deferred.clear();  // nothing left from a failed call
   RVSP_HIP_TRY(bs.t_vals.launch(s));
   const rvsp_csr_matrix_t Bst = view((int)bs.b_row_off.back(), bs.b_cols, bs.b_nnz_off.back(),
-                                     bs.b_rp.p, bs.b_ci.p, bs.b_val.p);
+                                     bs.b_rp.p, bs.b_ci.p, bs.b_vals());
   const bool conc = concurrent && bs.groups.size() > 1 && bs.lanes_ready();
   if (conc) {
     RVSP_HIP_TRY(hipEventRecord(bs.fork_ev, s));
@@ -903,7 +916,7 @@ rvsp_status_t rvsp_spgemm_batch_work_estimation(rvsp_spgemm_descr_t descr, int32
   // One inner descriptor per group over its rows of the stacked A.
   bs.c_nnz.assign(count, 0);
   const rvsp_csr_matrix_t Bst = view((int)bs.b_row_off[nb], bs.b_cols, bs.b_nnz_off[nb],
-                                     bs.b_rp.p, bs.b_ci.p, bs.b_val.p);
+                                     bs.b_rp.p, bs.b_ci.p, bs.b_vals());
   for (BatchGroup& g : bs.groups) {
     g.r0 = bs.a_row_off[g.p0];
     g.r1 = bs.a_row_off[g.p1];

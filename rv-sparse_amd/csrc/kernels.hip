@@ -1054,6 +1054,19 @@ hipError_t launch_fill_bin(const BinSpec& b, const int* rows, int nrows,
 // Numeric (reference: gustavson_core_f32.inc:18-64, accum_scalar_f32.c:19-71).
 
 // One nonempty B row: C(i, :) = 0 + a * B(k, :).
+// C's values leave the numeric kernels with non-temporal stores, so they do
+// not push B's values (gathered by later workgroups) out of the L2: on
+// 16-entry rows of C = A * A for uniform random A (4096 x 4096, 64
+// block-diagonal copies) that cut the position-map kernel from 165 to 130 us.
+// Shapes of at most two lanes per row keep ordinary stores, measured 1.7 %
+// faster on the 1024 x 1024 Laplacian, whose B rows are reread from the L2
+// anyway.
+template <int G>
+__device__ __forceinline__ void c_store(float v, float* dst) {
+  if constexpr (G > 2) __builtin_nontemporal_store(v, dst);
+  else *dst = v;
+}
+
 template <int G>
 __global__ __launch_bounds__(kBlock) void k_numeric_direct(
     const int* __restrict__ rows, int nrows, DevCsr A, DevCsr B,
@@ -1069,7 +1082,7 @@ __global__ __launch_bounds__(kBlock) void k_numeric_direct(
     const int k = A.ci[ap];
     const int bs = B.rp[k], be = B.rp[k + 1];
     const float a = A.val[ap];
-    for (int p = bs + l; p < be; p += G) out[p - bs] = acc_step(0.0f, a, B.val[p]);
+    for (int p = bs + l; p < be; p += G) c_store<G>(acc_step(0.0f, a, B.val[p]), out + (p - bs));
   }
 }
 
@@ -1238,10 +1251,10 @@ __global__ __launch_bounds__(kBlock) void k_numeric_search(
 
   if (seg) {
     __syncthreads();
-    for (int i = threadIdx.x; i < s1 - s0; i += kBlock) c_val[s0 + i] = vals[i];
+    for (int i = threadIdx.x; i < s1 - s0; i += kBlock) c_store<G>(vals[i], c_val + s0 + i);
   } else if (active) {
     const int c_s = c_rp[rows[r0 + g]];
-    for (int j = l; j < n; j += G) c_val[c_s + j] = rv[j];
+    for (int j = l; j < n; j += G) c_store<G>(rv[j], c_val + c_s + j);
   }
 }
 
@@ -1420,15 +1433,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(G <= 2 ?
   }
   extern __shared__ int lds[];
   constexpr int kGroups = kBlock / G;
-  constexpr int kMapCap = kGroups * kSegMapPerRow;  // map entries staged per segment
-  constexpr int kACap = kGroups * kSegAPerRow;      // A entries staged per segment
+  const int kMapCap = map_stage_cap_bytes(kGroups, cap) / (int)sizeof(T);  // map entries staged
+  const int kACap = kGroups * map_stage_a_per_row(cap);  // A entries staged per segment
   constexpr int kU = VAR == 2 ? 3 : 2;
   // Accumulators, then one 16-byte slot that the branch-free adds of lanes
   // without a product go to (VAR 2), the staged map slice and A metadata.
   float* vals = reinterpret_cast<float*>(lds);
   const int spare = kGroups * cap;  // that slot's index in vals
   T* mstage = reinterpret_cast<T*>(lds + kGroups * cap + 4);
-  float* m_a = reinterpret_cast<float*>(lds + kGroups * cap + 4 + map_stage_bytes(kGroups) / 4);
+  float* m_a = reinterpret_cast<float*>(lds + kGroups * cap + 4 + map_stage_bytes(kGroups, cap) / 4);
   int* m_bs = reinterpret_cast<int*>(m_a + kACap);
   int* m_be = m_bs + kACap;
   __shared__ int s_crp[kGroups + 1], s_arp[kGroups + 1];
@@ -1666,62 +1679,56 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(G <= 2 ?
         apply_chunk(map + mo);
     }
   } else {
-  // Load pipeline: entry ap's B values (kU per lane) in registers, entry
-  // ap+1's scale and bounds known, while entry ap is applied.
-  int bs0 = 0, be0 = 0, bs1 = 0, be1 = 0;
-  float a0 = 0.0f, a1 = 0.0f;
-  if (a_s < a_e) meta(a_s, &a0, &bs0, &be0);
-  if (a_s + 1 < a_e) meta(a_s + 1, &a1, &bs1, &be1);
-  float v0[kU], v1[kU];
-#pragma unroll
-  for (int j = 0; j < kU; ++j) {
-    const int p = bs0 + l + j * G;
-    v0[j] = p < be0 ? B.val[p] : 0.0f;
-  }
-
-  if (active) {
+  // Chunks of kC entries: their scales and B-row bounds (from LDS when
+  // staged), then the first kU B values of every entry of the chunk in
+  // flight together, then the entries applied in order. Entries past the
+  // row's last read the last one and are empty. (No load destination is
+  // copied into another register while in flight: such a copy waits for it.)
+  constexpr int kC = G <= 2 ? 4 : 8;
+  if (active && a_s < a_e) {
     float* rv = vals + base;
-    int t0 = 0;  // index of entry ap's first product within the row
-    for (int ap = a_s; ap < a_e; ++ap) {
+    int t0 = 0;  // index of the chunk's first product within the row
+    for (int c0 = a_s; c0 < a_e; c0 += kC) {
+      float av[kC], vv[kC][kU];
+      int bsv[kC], bev[kC];
 #pragma unroll
-      for (int j = 0; j < kU; ++j) {
-        const int p = bs1 + l + j * G;
-        v1[j] = p < be1 ? B.val[p] : 0.0f;
+      for (int u = 0; u < kC; ++u) {
+        meta(min(c0 + u, a_e - 1), &av[u], &bsv[u], &bev[u]);
+        if (c0 + u >= a_e) bev[u] = bsv[u];
       }
-      float a2 = 0.0f;
-      int bs2 = 0, be2 = 0;
-      if (ap + 2 < a_e) meta(ap + 2, &a2, &bs2, &be2);
-      const int len = be0 - bs0;
 #pragma unroll
-      for (int j = 0; j < kU; ++j) {
-        const int t = l + j * G;
-        if (t < len) {
-          const int pos = rm[t0 + t];
-          rv[pos] = acc_step(rv[pos], a0, v0[j]);
+      for (int u = 0; u < kC; ++u)
+#pragma unroll
+        for (int j = 0; j < kU; ++j) {
+          const int p = bsv[u] + l + j * G;
+          vv[u][j] = p < bev[u] ? B.val[p] : 0.0f;
         }
-      }
-      for (int t = kU * G + l; t < len; t += G) {
-        const int pos = rm[t0 + t];
-        rv[pos] = acc_step(rv[pos], a0, B.val[bs0 + t]);
-      }
-      t0 += len;
 #pragma unroll
-      for (int j = 0; j < kU; ++j) v0[j] = v1[j];
-      a0 = a1;
-      bs0 = bs1;
-      be0 = be1;
-      a1 = a2;
-      bs1 = bs2;
-      be1 = be2;
+      for (int u = 0; u < kC; ++u) {
+        const int len = bev[u] - bsv[u];
+#pragma unroll
+        for (int j = 0; j < kU; ++j) {
+          const int t = l + j * G;
+          if (t < len) {
+            const int pos = rm[t0 + t];
+            rv[pos] = acc_step(rv[pos], av[u], vv[u][j]);
+          }
+        }
+        for (int t = kU * G + l; t < len; t += G) {
+          const int pos = rm[t0 + t];
+          rv[pos] = acc_step(rv[pos], av[u], B.val[bsv[u] + t]);
+        }
+        t0 += len;
+      }
     }
   }
   }  // VAR != 2
 
   if (seg) {
     __syncthreads();
-    for (int i = threadIdx.x; i < s1 - s0; i += kBlock) c_val[s0 + i] = vals[i];
+    for (int i = threadIdx.x; i < s1 - s0; i += kBlock) c_store<G>(vals[i], c_val + s0 + i);
   } else {
-    for (int j = l; j < n; j += G) c_val[c_s + j] = vals[base + j];
+    for (int j = l; j < n; j += G) c_store<G>(vals[base + j], c_val + c_s + j);
   }
 }
 
@@ -1838,7 +1845,7 @@ __global__ __launch_bounds__(kBlock) void k_numeric_rowblock(
         }
       }
       for (int j = p0 + lane; j < p1; j += kWave)
-        c_val[c_s + j] = GLOBAL_ACC ? acc[c_ci[c_s + j]] : vals[j];
+        __builtin_nontemporal_store(GLOBAL_ACC ? acc[c_ci[c_s + j]] : vals[j], c_val + c_s + j);
     }
     __syncthreads();  // LDS / slot reuse by the next row
   }
@@ -2254,7 +2261,8 @@ __device__ __forceinline__ void tiled_row(const TiledLds& L, int* scan_tmp, int 
         }
         __syncthreads();  // the chunk's metadata is replaced next
       }
-      for (int j = tid; j < tn; j += kBlock) c_val[c_s + p0 + j] = L.vals[j];
+      for (int j = tid; j < tn; j += kBlock)
+        __builtin_nontemporal_store(L.vals[j], c_val + c_s + p0 + j);
       __syncthreads();  // the tile's LDS is reused by the next tile
       p0 = pe;
       first = false;
@@ -2545,7 +2553,8 @@ __global__ __launch_bounds__(kWave) void k_numeric_wave(
           }
         }
       }
-      for (int j = lane; j < tn; j += kWave) c_val[c_s + p0 + j] = vals[j];
+      for (int j = lane; j < tn; j += kWave)
+        __builtin_nontemporal_store(vals[j], c_val + c_s + p0 + j);
       p0 = pe;
       first = false;
     }

@@ -372,10 +372,23 @@ constexpr int kStaticLdsReserve = 16 * 1024;
 // workgroup whose rows hold more A entries reads them from global memory).
 constexpr int kSegAPerRow = 6;
 
-// Map entries per row the MAP kernel's segment mode stages in LDS.
-constexpr int kSegMapPerRow = 32;
-// LDS bytes of a segment's staged map entries (2-byte entries at most, plus
-// room for the 16-byte alignment of the staging loads).
-constexpr int map_stage_bytes(int groups) { return (groups * kSegMapPerRow * 2 + 32 + 15) / 16 * 16; }
+// Bytes of map entries the MAP kernel's segment mode stages in LDS: per row
+// the bin's accumulator capacity, within 64..256 bytes, and at most 8 KB per
+// workgroup (a segment whose entries need more reads them from global
+// memory). 1-byte maps hold twice the entries of 2-byte ones.
+__host__ __device__ constexpr int map_stage_cap_bytes(int groups, int cap) {
+  const int per_row = cap < 64 ? 64 : (cap > 256 ? 256 : cap);
+  return groups * per_row < 8192 ? groups * per_row : 8192;
+}
+// LDS bytes of a segment's staged map entries, with room for the 16-byte
+// alignment of the staging loads.
+__host__ __device__ constexpr int map_stage_bytes(int groups, int cap) {
+  return (map_stage_cap_bytes(groups, cap) + 32 + 15) / 16 * 16;
+}
+// A entries per row whose scale and B-row bounds the MAP kernel's variant 1
+// stages in LDS (12 bytes each).
+__host__ __device__ constexpr int map_stage_a_per_row(int cap) {
+  return cap / 16 < kSegAPerRow ? kSegAPerRow : (cap / 16 > 16 ? 16 : cap / 16);
+}
 
 }  // namespace rvsp

@@ -75,8 +75,8 @@ int lds_bytes_for(int phase, int kind, int group, int cap, int variant) {
       if (phase != PHASE_NUMERIC || !sub_wave || cap < 1 || cap > 65536) break;
       // Accumulators, a 16-byte spare slot, and the staged slice of the map
       // (up to 2-byte entries).
-      bytes = (long long)(kBlock / group) * cap * 4 + 16 + map_stage_bytes(kBlock / group);
-      if (variant == 1) bytes += (long long)(kBlock / group) * kSegAPerRow * 12;
+      bytes = (long long)(kBlock / group) * cap * 4 + 16 + map_stage_bytes(kBlock / group, cap);
+      if (variant == 1) bytes += (long long)(kBlock / group) * map_stage_a_per_row(cap) * 12;
       else if (variant != 0 && variant != 2) bytes = -1;
       break;
     case KIND_BITMAP:
