@@ -1408,14 +1408,17 @@ hipError_t launch_build_map(int width, const int* rows, int nrows, const DevCsr&
   return hipGetLastError();
 }
 
-// Numeric pass driven by the position map. Same row/group structure,
-// segment mode and load pipeline as k_numeric_search, but LDS holds only the
-// accumulators (and, in segment mode, the segment's slice of the map), and
-// each product is one map entry, one B value and one LDS add. VAR 1 also
+// Numeric pass driven by the position map. Same row/group structure and
+// segment mode as k_numeric_search, but LDS holds only the accumulators
+// (and, in segment mode, the segment's slice of the map, up to
+// map_stage_cap_bytes), and each product is one map entry, one B value and
+// one LDS add. VAR 0 and VAR 1 walk a row's A entries in chunks of kC, the
+// first kU B values of every entry of a chunk in flight together; VAR 1 also
 // stages every A entry's scale and B-row bounds of the segment in LDS, which
 // removes the per-group (G-fold redundant) scattered loads of A and B's row
-// pointers at the cost of LDS and one more load round before the barrier
-// (VAR 1). VAR 2 instead fetches A entries in chunks (see below).
+// pointers at the cost of LDS and one more load round before the barrier.
+// VAR 2 fetches the A entries of a chunk with one load per lane and shares
+// them by lane shuffles (see below).
 template <int G, typename T, int VAR>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(G <= 2 ? 8 : 1))) void k_numeric_map(
     const int* __restrict__ rows, int nrows, int row_base, int cap, DevCsr A, DevCsr B,
