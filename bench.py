@@ -1019,7 +1019,7 @@ def main():
     ap.add_argument("--batch-algo", choices=["fixed", "learned"], default="learned")
     ap.add_argument("--rmat-scale", type=int, default=20)
     ap.add_argument("--banded-runs", type=int, default=3)
-    ap.add_argument("--powerlaw-runs", type=int, default=3)
+    ap.add_argument("--powerlaw-runs", type=int, default=5)
     ap.add_argument("--powerlaw-plans", default=None,
                     help="comma list of plain_fixed, batched_fixed, learned_batched "
                          "(profiling runs; default all)")
