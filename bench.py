@@ -626,8 +626,12 @@ def bench_powerlaw(hip, args, F):
         return t1 - t0, t2 - t1, groups
 
     def sums():
-        return (word_checksum(ci_out[:total]), word_checksum(val_out[:total]),
-                sum(word_checksum(r) for r in c_rps) % (1 << 64))
+        out = (word_checksum(ci_out[:total]), word_checksum(val_out[:total]),
+               sum(word_checksum(r) for r in c_rps) % (1 << 64))
+        # The checksums' temporaries go back to the device, so the next
+        # plan's allocations find the memory the earlier runs left them.
+        torch.cuda.empty_cache()
+        return out
 
     plans = {"plain_fixed": run_plain,
              "batched_fixed": lambda: run_batched(F.ALGO_HIP_FIXED),
