@@ -232,7 +232,7 @@ rvsp_status_t rvsp_hip_synchronize(void);
 /* Descriptors take their device state from a cache of released buffers, so
  * repeated estimations do not pay for fresh multi-GB device allocations
  * (no counterpart in the reference, which allocates host memory per call).
- * The cache holds at most a quarter of the device's memory (environment
+ * The cache holds at most half of the device's memory (environment
  * variable RVSP_POOL_CAP_MB overrides) and is emptied when an allocation
  * runs out of memory. rvsp_hip_pool_release returns every cached buffer to
  * the device; rvsp_hip_pool_cached_bytes reports how much is held. */

@@ -66,7 +66,11 @@ size_t cache_cap_locked(Pool& p) {
       (void)hipGetLastError();
       total_b = (size_t)64 << 30;
     }
-    p.cap = total_b / 4;
+    // Half the device: the working set of a batched product over many
+    // groups (R-MAT 2^20 in 10 groups keeps about 100 GB of sorted columns
+    // between its count and fill phases) must fit, or every run frees and
+    // re-allocates its largest buffers.
+    p.cap = total_b / 2;
     if (const char* env = getenv("RVSP_POOL_CAP_MB")) p.cap = (size_t)atoll(env) << 20;
   }
   return p.cap;
