@@ -162,6 +162,37 @@ def test_repeated_numeric_computes_with_new_values(gpu, graph):
             d.free()
 
 
+@pytest.mark.parametrize("graph", [True, False])
+def test_square_pairs_share_one_value_array(gpu, graph):
+    """C = A*A for every pair through one device matrix per pair (its value
+    array is both operands' values): the batch stages only A's values, and
+    new values in that array reach both operands on every compute."""
+    mats = [gen.uniform_k(n, 8, seed=s) for s, n in enumerate((64, 300, 1024, 2048, 513))]
+    dmats = [rvsp.DeviceCsr.upload(M) for M in mats]
+    bt = Batch(graph=graph)
+    out = None
+    try:
+        nnz = bt.estimate(dmats, dmats)
+        out = BatchOutput([M.rows for M in mats], [M.cols for M in mats], nnz)
+        rng = np.random.default_rng(11)
+        for it in range(4):
+            if it:
+                mats = [M.with_values(rng.uniform(0.5, 1.5, M.nnz).astype(np.float32))
+                        for M in mats]
+                for M, dM in zip(mats, dmats):
+                    dM.set_values(M.values)
+            F.check(bt.compute(dmats, dmats, out.structs), "batch_compute")
+            assert_pairs_match([(M, M) for M in mats],
+                               [out.download(p) for p in range(len(mats))])
+        assert bt.graph_launches() == (2 if graph else 0)
+    finally:
+        if out is not None:
+            out.free()
+        bt.close()
+        for d in dmats:
+            d.free()
+
+
 def test_batch_edge_pairs(gpu):
     """Pairs with empty rows, an all-empty product and 1x1 operands."""
     empty = Csr(5, 7, np.zeros(6, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32))
